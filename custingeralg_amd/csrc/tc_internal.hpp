@@ -1,0 +1,140 @@
+// tc_internal.hpp -- shared host/device definitions of the triangle-counting
+// library (not part of the public C-ABI; see include/custinger_tc.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/custinger_tc.h"
+
+namespace tc {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors --
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_last_error(int code, const std::string &msg);
+
+#define TC_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ::tc::set_last_error(TC_ERR_HIP, std::string(#call) + ": " +               \
+                                                 hipGetErrorString(_e));               \
+            return TC_ERR_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+#define TC_TRY(expr)                                                                   \
+    do {                                                                               \
+        int _s = (expr);                                                               \
+        if (_s != TC_OK) return _s;                                                    \
+    } while (0)
+
+// ------------------------------------------------------- device buffers --
+// Scratch allocations are owned by the graph handle and grown on demand.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ------------------------------------------------------------- helpers --
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline int grid_for(int64_t n, int block, int64_t cap = 1 << 20)
+{
+    int64_t g = ceil_div(n, block);
+    if (g < 1) g = 1;
+    if (g > cap) g = (int)cap;
+    return (int)g;
+}
+
+// Device-wide exclusive scan of int64 values (in may equal out).  total, if
+// non-null, receives the sum on the device (one int64).  Uses tmp scratch.
+int scan_exclusive_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total,
+                       DevBuf &tmp, hipStream_t s);
+// int32 counts -> int64 exclusive prefix (out has n+1 entries, out[n]=sum).
+int scan_counts_i32(const int32_t *in, int64_t *out, int64_t n, DevBuf &tmp, hipStream_t s);
+
+// LSD radix sort of uint64 keys over bits [0, bits).  Keys and an optional
+// int32 payload are sorted in place (alt buffers supplied by scratch).
+int radix_sort_u64(uint64_t *keys, int32_t *vals, int64_t n, int bits, DevBuf &alt_keys,
+                   DevBuf &alt_vals, DevBuf &tmp, hipStream_t s);
+
+// ------------------------------------------------------- vertex views --
+// One 16-byte record per vertex used by every intersection kernel:
+// the list of v is adj[rb .. rb+len); in the degree-partitioned layout the
+// first dm entries are N-(v) (lower rank) and the rest N+(v).
+struct VInfo {
+    int64_t rb;
+    int32_t dm;
+    int32_t len;
+};
+
+// Intersection jobs: a slice [t_begin, t_end) of an owner's task list and the
+// tile of the owner's list that sits in LDS.
+struct Job {
+    int32_t owner;
+    int32_t t_begin;
+    int32_t t_end;
+    int32_t tile;
+};
+
+// ------------------------------------------------------------- store --
+// Dynamic adjacency store: vertex v owns a block pool[boff[v] .. boff[v]+cap[v])
+// of which the first len[v] entries are used, sorted ascending.
+struct Store {
+    int32_t nv = 0;
+    int64_t nnz = 0;           // sum of len
+    int64_t pool_used = 0;     // high-water mark of allocated blocks
+    int64_t reallocs = 0;
+    DevBuf boff, len, cap, pool;
+};
+
+struct Graph {
+    Store st;
+    hipStream_t stream = nullptr;
+    bool validated = false;
+    int symmetric = -1;        // -1 unknown
+    int64_t self_loops = -1;   // -1 unknown
+    int32_t max_degree = 0;
+    // cached derived structures (rebuilt after updates)
+    DevBuf vinfo, padj, coff, work, jobs, tmp, tmp2, tmp3, tmp4, accum;
+    DevBuf bsrc, bdst, bflag, bkeys, bkeys2, bvals2, boffb, bdel;
+    hipEvent_t ev[16] = {};
+    bool events_ready = false;
+    int ensure_events();
+};
+
+// Fills vinfo with the store's rows as given (direct layout, dm=0).
+int build_direct_vinfo(Graph &g);
+// Validation of sortedness/range, self-loop count, optional symmetry check.
+int validate_graph(Graph &g, bool check_symmetry, int64_t *bad_rows);
+
+int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &sched,
+                 tc_stats *stats, bool skip_lower = false);
+int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t *adj, int location,
+                     uint32_t flags, int32_t slack_pct, Graph **out);
+int graph_to_csr(Graph &g, int64_t *off, int32_t *adj, int location);
+void graph_destroy(Graph *g);
+tc_schedule default_schedule();
+
+// Hash used by the LDS tables (multiplicative, table size 2^bits).
+__host__ __device__ inline uint32_t lds_hash(int32_t w, int bits)
+{
+    return ((uint32_t)w * 0x9E3779B1u) >> (32 - bits);
+}
+
+}  // namespace tc

@@ -1,0 +1,848 @@
+// tc_static.hip -- static triangle counting on MI355X (gfx950).
+//
+// Replaces devicecuStingerAllTriangles (src/static_triangle_counting/cct.cu:216-270)
+// with the same per-vertex result and a different method:
+//
+//  * ORIENTED (symmetric, loop-free graphs).  Rank vertices by (degree, id) and
+//    split every row into N-(v) (lower rank) and N+(v) (higher rank), each
+//    still id-sorted.  A triangle a<b<w (rank order) is found exactly once, by
+//    the job that owns b: N+(b) sits in LDS (sorted array + hash of
+//    positions), and every task a in N-(b) intersects N+(a) with it.  A hit w
+//    adds 1 to an LDS counter of w's slot and to the task's counter, so the
+//    per-vertex result 2*t(v) needs at most one global atomic per (job, vertex)
+//    instead of one per triangle.
+//  * DIRECT (any validated input).  Every entry (a,b) of a row is a task,
+//    |N(a) cap N(b)| with the reference's filters (cct.cu:247-249) is added to
+//    out[a] only -- the reference's own formula.
+//
+// Both run on the same kernel: owner list in LDS, tasks flattened across the
+// 64 lanes of a wavefront, three strategies per task (lane-sequential merge,
+// binary search of the owner list into the task's list, streaming the task's
+// list through the LDS hash), chosen by the fixed degree-ratio heuristic or
+// the learned table.  Jobs come from a prefix-sum partition of the estimated
+// work per owner (the load-balance layer, include/load_balance.cuh:77-102),
+// binned by owner-list length (small / mid / large+tiled).
+#include "tc_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace tc {
+
+enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2 };
+
+// ------------------------------------------------------------ helpers --
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline int32_t wave_incl_scan_i32(int32_t x)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        int32_t y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+template <class T> __device__ inline T wave_sum(T x)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+__host__ __device__ inline int ceil_log2_i(int64_t x)
+{
+    int r = 0;
+    while ((int64_t(1) << r) < x) ++r;
+    return r;
+}
+
+// Largest r in [lo,hi] with coff[r] <= e.
+__device__ inline int32_t row_search(const int64_t *__restrict__ coff, int64_t e, int32_t lo,
+                                     int32_t hi)
+{
+    while (lo < hi) {
+        int32_t mid = lo + ((hi - lo + 1) >> 1);
+        if (coff[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Rows of the 64 entries [e0, e0+64) of a wave (entries past nnz get the
+// last row; callers mask them).
+__device__ inline int32_t wave_row_of(const int64_t *__restrict__ coff, int32_t nv, int64_t e0,
+                                      int64_t nnz)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    int64_t e = e0 + lane;
+    int64_t elast = min(e0 + kWave, nnz) - 1;
+    int32_t rfirst = 0, rlast = 0;
+    if (lane == 0) rfirst = row_search(coff, e0, 0, nv - 1);
+    if (lane == 1) rlast = row_search(coff, elast, 0, nv - 1);
+    rfirst = __shfl(rfirst, 0, kWave);
+    rlast = __shfl(rlast, 1, kWave);
+    if (e > elast) e = elast;
+    return row_search(coff, e, rfirst, rlast);
+}
+
+__device__ inline bool rank_less(int32_t a, int32_t la, int32_t b, int32_t lb)
+{
+    return (la < lb) || (la == lb && a < b);
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// ---------------------------------------------------------- orientation --
+
+// mask[c] = bit i set iff compact entry 64c+i points to a higher-rank vertex.
+__global__ void k_orient_mask(const int64_t *__restrict__ coff, const int64_t *__restrict__ boff,
+                              const int32_t *__restrict__ len, const int32_t *__restrict__ pool,
+                              int32_t nv, int64_t nnz, uint64_t *__restrict__ mask,
+                              int32_t *__restrict__ cnt)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        int64_t e = c * kWave + lane;
+        int32_t a = wave_row_of(coff, nv, c * kWave, nnz);
+        bool plus = false;
+        if (e < nnz) {
+            int32_t b = pool[boff[a] + (e - coff[a])];
+            plus = rank_less(a, len[a], b, len[b]);
+        }
+        uint64_t m = __ballot(plus);
+        if (lane == 0) {
+            mask[c] = m;
+            cnt[c] = __popcll(m);
+        }
+    }
+}
+
+__device__ inline int64_t plus_prefix(const int64_t *__restrict__ cscan,
+                                      const uint64_t *__restrict__ mask, int64_t x)
+{
+    int64_t c = x >> 6;
+    int r = (int)(x & 63);
+    uint64_t m = r ? (mask[c] & (~0ull >> (64 - r))) : 0ull;
+    return cscan[c] + __popcll(m);
+}
+
+__global__ void k_orient_vinfo(const int64_t *__restrict__ coff, const int32_t *__restrict__ len,
+                               const int64_t *__restrict__ cscan, const uint64_t *__restrict__ mask,
+                               int32_t nv, VInfo *__restrict__ vi, int32_t *__restrict__ maxplus)
+{
+    int32_t mp = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p0 = plus_prefix(cscan, mask, coff[v]);
+        int64_t p1 = plus_prefix(cscan, mask, coff[v + 1]);
+        int32_t dp = (int32_t)(p1 - p0);
+        VInfo x;
+        x.rb = coff[v];
+        x.len = len[v];
+        x.dm = x.len - dp;
+        vi[v] = x;
+        mp = max(mp, dp);
+    }
+    mp = max(mp, __shfl_xor(mp, 1));
+    for (int d = 2; d < kWave; d <<= 1) mp = max(mp, __shfl_xor(mp, d));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxplus, mp);
+}
+
+__global__ void k_orient_write(const int64_t *__restrict__ coff, const int64_t *__restrict__ boff,
+                               const int32_t *__restrict__ pool, const int64_t *__restrict__ cscan,
+                               const uint64_t *__restrict__ mask, const VInfo *__restrict__ vi,
+                               int32_t nv, int64_t nnz, int32_t *__restrict__ padj)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        int64_t e = c * kWave + lane;
+        int32_t a = wave_row_of(coff, nv, c * kWave, nnz);
+        if (e >= nnz) continue;
+        uint64_t m = mask[c];
+        bool plus = (m >> lane) & 1ull;
+        int64_t pe = cscan[c] + __popcll(m & lt);
+        int64_t ca = coff[a];
+        int64_t rp = pe - plus_prefix(cscan, mask, ca);
+        int64_t k = e - ca;
+        int32_t b = pool[boff[a] + k];
+        int64_t dst = plus ? (ca + vi[a].dm + rp) : (ca + (k - rp));
+        padj[dst] = b;
+    }
+}
+
+__global__ void k_direct_vinfo(const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
+                               int32_t nv, VInfo *__restrict__ vi)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        VInfo x;
+        x.rb = boff[v];
+        x.len = len[v];
+        x.dm = 0;
+        vi[v] = x;
+    }
+}
+
+// ------------------------------------------------------------- planning --
+
+struct PlanParams {
+    int32_t lmax[3];     // owner-list capacity of each bin's kernel
+    int32_t bin_max[2];  // |owner| <= bin_max[0] -> bin 0, <= bin_max[1] -> bin 1, else 2
+    int32_t job_items;   // target estimated items per job
+};
+
+__device__ inline int32_t est_items(int32_t L, int32_t pl)
+{
+    // cheapest of streaming the task list (pl) and searching the owner list (L*log pl)
+    int32_t lg = 1;
+    while ((1 << lg) < pl + 1) ++lg;
+    int64_t bs = (int64_t)L * lg;
+    return (int32_t)min<int64_t>(pl, bs) + 4;
+}
+
+__device__ inline int owner_bin(int32_t L, const PlanParams &P)
+{
+    return L <= P.bin_max[0] ? 0 : (L <= P.bin_max[1] ? 1 : 2);
+}
+
+// Per-owner estimated work (sum over its tasks); per-bin stats.
+template <bool ORIENTED>
+__global__ void k_owner_work(const int64_t *__restrict__ coff, const VInfo *__restrict__ vi,
+                             const int32_t *__restrict__ adj, int32_t nv, int64_t nnz,
+                             PlanParams P, unsigned long long *__restrict__ work,
+                             unsigned long long *__restrict__ bstats)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        int64_t e = c * kWave + lane;
+        int32_t b = wave_row_of(coff, nv, c * kWave, nnz);
+        unsigned long long w = 0, bytes = 0;
+        int bin = 0;
+        bool task = false;
+        if (e < nnz) {
+            VInfo vb = vi[b];
+            int64_t k = e - coff[b];
+            int32_t L = ORIENTED ? vb.len - vb.dm : vb.len;
+            bool is_task_entry = ORIENTED ? (k < vb.dm) : true;
+            if (is_task_entry && L > 0) {
+                int32_t a = adj[vb.rb + k];
+                VInfo va = vi[a];
+                int32_t pl = ORIENTED ? va.len - va.dm : va.len;
+                bool ok = ORIENTED ? (pl > 0) : (a != b && va.len >= 2 && vb.len >= 2);
+                if (ok) {
+                    task = true;
+                    w = (unsigned long long)est_items(min(L, P.lmax[2]), pl);
+                    bytes = 4ull * (unsigned long long)(L + pl);
+                    bin = owner_bin(L, P);
+                }
+            }
+        }
+        // segmented (by owner row) inclusive scan of w across the wave
+        unsigned long long x = w;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            unsigned long long y = __shfl_up(x, d, kWave);
+            int32_t rb = __shfl_up(b, d, kWave);
+            if (lane >= d && rb == b) x += y;
+        }
+        int32_t bnext = __shfl_down(b, 1, kWave);
+        bool tail = (lane == kWave - 1) || (bnext != b) || (e + 1 >= nnz);
+        if (e < nnz && tail && x) atomicAdd(&work[b], x);
+        if (bstats) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                unsigned long long t = wave_sum<unsigned long long>((task && bin == q) ? 1ull : 0ull);
+                unsigned long long by = wave_sum<unsigned long long>((task && bin == q) ? bytes : 0ull);
+                if (lane == 0 && t) {
+                    atomicAdd(&bstats[2 * q], t);
+                    atomicAdd(&bstats[2 * q + 1], by);
+                }
+            }
+        }
+    }
+}
+
+template <bool ORIENTED>
+__global__ void k_plan_jobs(const VInfo *__restrict__ vi, const unsigned long long *__restrict__ work,
+                            int32_t nv, PlanParams P, int32_t *__restrict__ njobs /* [3][nv] */)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        VInfo x = vi[v];
+        int32_t L = ORIENTED ? x.len - x.dm : x.len;
+        int32_t T = ORIENTED ? x.dm : x.len;
+        int32_t nj[3] = {0, 0, 0};
+        unsigned long long w = work[v];
+        if (L > 0 && T > 0 && w > 0 && (ORIENTED || x.len >= 2)) {
+            int bin = owner_bin(L, P);
+            int64_t tiles = ceil_div(L, P.lmax[bin]);
+            int64_t per_tile = (int64_t)(w / (unsigned long long)tiles) + 1;
+            int64_t split = ceil_div(per_tile, P.job_items);
+            if (split > T) split = T;
+            if (split < 1) split = 1;
+            nj[bin] = (int32_t)(split * tiles);
+        }
+        njobs[v] = nj[0];
+        njobs[(int64_t)nv + v] = nj[1];
+        njobs[2 * (int64_t)nv + v] = nj[2];
+    }
+}
+
+template <bool ORIENTED>
+__global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restrict__ njobs,
+                            const int64_t *__restrict__ jscan /* [3][nv+1] */, int32_t nv,
+                            PlanParams P, Job *__restrict__ jobs0, Job *__restrict__ jobs1,
+                            Job *__restrict__ jobs2)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        VInfo x = vi[v];
+        int32_t L = ORIENTED ? x.len - x.dm : x.len;
+        int32_t T = ORIENTED ? x.dm : x.len;
+        for (int bin = 0; bin < 3; ++bin) {
+            int32_t nj = njobs[(int64_t)bin * nv + v];
+            if (!nj) continue;
+            Job *out = bin == 0 ? jobs0 : (bin == 1 ? jobs1 : jobs2);
+            int64_t base = jscan[(int64_t)bin * (nv + 1) + v];
+            int32_t tiles = (int32_t)ceil_div(L, P.lmax[bin]);
+            int32_t split = nj / tiles;
+            int64_t q = 0;
+            for (int32_t t = 0; t < tiles; ++t)
+                for (int32_t s = 0; s < split; ++s) {
+                    Job j;
+                    j.owner = (int32_t)v;
+                    j.t_begin = (int32_t)(((int64_t)T * s) / split);
+                    j.t_end = (int32_t)(((int64_t)T * (s + 1)) / split);
+                    j.tile = t;
+                    out[base + q++] = j;
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------- count kernel --
+
+struct CountArgs {
+    const int32_t *adj;
+    const VInfo *vi;
+    uint32_t *out;
+    unsigned long long *global;
+    unsigned long long *strat;  // [3] tasks per strategy (may be null)
+    const uint8_t *table;       // learned strategy table (policy LEARNED)
+    int32_t table_dim;
+    int32_t policy;
+    int32_t ratio_x16;
+    int32_t skip_lower;  // CSR convention: entries with dest < src are skipped
+};
+
+__device__ inline int lg_bucket(int32_t x)
+{
+    // floor(log2(x)) + 1 for x >= 1, 0 for x == 0
+    return x <= 0 ? 0 : 32 - __clz(x);
+}
+
+__device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
+{
+    switch (A.policy) {
+    case TC_POLICY_MERGE: return S_MERGE;
+    case TC_POLICY_BSEARCH: return S_BSEARCH;
+    case TC_POLICY_HASH: return S_HASH;
+    case TC_POLICY_LEARNED:
+        if (A.table) {
+            int i = min(lg_bucket(L), A.table_dim - 1);
+            int j = min(lg_bucket(pl), A.table_dim - 1);
+            return A.table[i * A.table_dim + j];
+        }
+        break;
+    default: break;
+    }
+    // fixed degree-ratio heuristic
+    int32_t sh = min(L, pl), lo = max(L, pl);
+    if ((int64_t)lo * 16 <= (int64_t)sh * A.ratio_x16 && lo <= 96) return S_MERGE;
+    // searching the owner list into a much longer task list beats streaming it
+    if ((int64_t)L * lg_bucket(pl) * 4 < (int64_t)pl) return S_BSEARCH;
+    return S_HASH;
+}
+
+template <int LMAX, int NW, bool ORIENTED>
+__global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jobs, int64_t njobs,
+                                                      CountArgs A)
+{
+    constexpr int HMAX = 2 * LMAX;
+    constexpr int HBMAX = (HMAX <= 1) ? 1 : 32 - __builtin_clz(HMAX - 1);
+    __shared__ int32_t S[LMAX];
+    __shared__ int32_t H[HMAX];
+    __shared__ uint32_t C[ORIENTED ? LMAX : 1];
+    __shared__ int32_t pre[NW][kWave + 1];
+    __shared__ int64_t tpb[NW][kWave];
+    __shared__ int32_t tpl[NW][kWave];
+    __shared__ uint32_t tcnt[NW][kWave];
+    __shared__ unsigned long long red[NW];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nthr = NW * kWave;
+    unsigned long long hits_all = 0;
+    unsigned long long sc[3] = {0, 0, 0};
+
+    for (int64_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+        const Job J = jobs[jb];
+        const VInfo vo = A.vi[J.owner];
+        const int64_t obase = ORIENTED ? vo.rb + vo.dm : vo.rb;
+        const int32_t Lfull = ORIENTED ? vo.len - vo.dm : vo.len;
+        const int32_t s0 = J.tile * LMAX;
+        const int32_t L = min(LMAX, Lfull - s0);
+        const bool tiled = Lfull > LMAX;
+        int hb = 1;
+        while ((1 << hb) < 2 * L) ++hb;
+        if (hb > HBMAX) hb = HBMAX;
+        const int32_t HS = 1 << hb;
+        for (int i = threadIdx.x; i < L; i += nthr) S[i] = A.adj[obase + s0 + i];
+        for (int i = threadIdx.x; i < HS; i += nthr) H[i] = -1;
+        if (ORIENTED)
+            for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < L; i += nthr) {
+            uint32_t slot = lds_hash(S[i], hb);
+            while (atomicCAS(&H[slot], -1, i) != -1) slot = (slot + 1) & (HS - 1);
+        }
+        __syncthreads();
+        const int32_t lo_v = S[0], hi_v = S[L - 1];
+        unsigned long long hits_job = 0;
+
+        for (int32_t t0 = J.t_begin + wid * kWave; t0 < J.t_end; t0 += NW * kWave) {
+            const int32_t t = t0 + lane;
+            bool valid = t < J.t_end;
+            int32_t tv = valid ? A.adj[vo.rb + t] : 0;
+            VInfo vt = {0, 0, 0};
+            if (valid) vt = A.vi[tv];
+            int64_t pb = ORIENTED ? vt.rb + vt.dm : vt.rb;
+            int32_t pl = ORIENTED ? vt.len - vt.dm : vt.len;
+            if (!ORIENTED && valid &&
+                (tv == J.owner || vt.len < 2 || vo.len < 2 || (A.skip_lower && tv < J.owner)))
+                valid = false;
+            if (!valid) pl = 0;
+            if (tiled && pl > 0) {
+                // keep only the part of the task list inside the owner tile's range
+                int32_t lo = 0, hi = pl;
+                while (lo < hi) {
+                    int32_t mid = (lo + hi) >> 1;
+                    if (A.adj[pb + mid] < lo_v) lo = mid + 1; else hi = mid;
+                }
+                int32_t a0 = lo;
+                hi = pl;
+                while (lo < hi) {
+                    int32_t mid = (lo + hi) >> 1;
+                    if (A.adj[pb + mid] <= hi_v) lo = mid + 1; else hi = mid;
+                }
+                pb += a0;
+                pl = lo - a0;
+            }
+            const int strat = (pl > 0) ? choose_strategy(L, pl, A) : -1;
+            if (strat >= 0) sc[strat]++;
+            tpb[wid][lane] = pb;
+            tpl[wid][lane] = pl;
+            tcnt[wid][lane] = 0;
+            uint32_t mcount = 0;
+            wave_sync();
+
+            // -- hash: stream the task list through the LDS hash of the owner list
+            {
+                int32_t items = (strat == S_HASH) ? pl : 0;
+                int32_t incl = wave_incl_scan_i32(items);
+                pre[wid][lane + 1] = incl;
+                if (lane == 0) pre[wid][0] = 0;
+                const int32_t total = __shfl(incl, kWave - 1, kWave);
+                wave_sync();
+                for (int32_t it = lane; it < total; it += kWave) {
+                    int32_t lo = 0, hi = kWave;
+                    while (hi - lo > 1) {
+                        int32_t mid = (lo + hi) >> 1;
+                        if (pre[wid][mid] <= it) lo = mid; else hi = mid;
+                    }
+                    const int32_t j = lo;
+                    const int32_t k = it - pre[wid][j];
+                    const int32_t w = A.adj[tpb[wid][j] + k];
+                    uint32_t slot = lds_hash(w, hb);
+                    int32_t pos;
+                    while (true) {
+                        pos = H[slot];
+                        if (pos < 0 || S[pos] == w) break;
+                        slot = (slot + 1) & (HS - 1);
+                    }
+                    if (pos >= 0) {
+                        atomicAdd(&tcnt[wid][j], 1u);
+                        if (ORIENTED) atomicAdd(&C[pos], 1u);
+                    }
+                }
+            }
+            // -- bsearch: each owner-list element searched in the task list
+            {
+                int32_t items = (strat == S_BSEARCH) ? L : 0;
+                int32_t incl = wave_incl_scan_i32(items);
+                wave_sync();
+                pre[wid][lane + 1] = incl;
+                if (lane == 0) pre[wid][0] = 0;
+                const int32_t total = __shfl(incl, kWave - 1, kWave);
+                wave_sync();
+                for (int32_t it = lane; it < total; it += kWave) {
+                    int32_t lo = 0, hi = kWave;
+                    while (hi - lo > 1) {
+                        int32_t mid = (lo + hi) >> 1;
+                        if (pre[wid][mid] <= it) lo = mid; else hi = mid;
+                    }
+                    const int32_t j = lo;
+                    const int32_t k = it - pre[wid][j];
+                    const int32_t w = S[k];
+                    const int32_t *lst = A.adj + tpb[wid][j];
+                    int32_t a = 0, b = tpl[wid][j];
+                    while (a < b) {
+                        int32_t mid = (a + b) >> 1;
+                        if (lst[mid] < w) a = mid + 1; else b = mid;
+                    }
+                    if (a < tpl[wid][j] && lst[a] == w) {
+                        atomicAdd(&tcnt[wid][j], 1u);
+                        if (ORIENTED) atomicAdd(&C[k], 1u);
+                    }
+                }
+            }
+            // -- merge: one lane walks both sorted lists
+            if (strat == S_MERGE) {
+                const int32_t *lst = A.adj + pb;
+                int32_t i = 0, k = 0;
+                // skip owner entries below the task list's first element
+                {
+                    int32_t first = lst[0], a = 0, b = L;
+                    while (a < b) {
+                        int32_t mid = (a + b) >> 1;
+                        if (S[mid] < first) a = mid + 1; else b = mid;
+                    }
+                    k = a;
+                }
+                while (i < pl && k < L) {
+                    int32_t x = lst[i], y = S[k];
+                    if (x == y) {
+                        ++mcount;
+                        if (ORIENTED) atomicAdd(&C[k], 1u);
+                    }
+                    i += (x <= y);
+                    k += (x >= y);
+                }
+            }
+            wave_sync();
+            const uint32_t c = tcnt[wid][lane] + mcount;
+            if (valid && c) {
+                hits_job += c;
+                if (ORIENTED) atomicAdd(&A.out[tv], 2u * c);
+            }
+            wave_sync();
+        }
+        // owner + third-vertex flush
+        unsigned long long hj = wave_sum<unsigned long long>(hits_job);
+        if (lane == 0) red[wid] = hj;
+        __syncthreads();
+        unsigned long long hjob = 0;
+        for (int w = 0; w < NW; ++w) hjob += red[w];
+        if (ORIENTED) {
+            for (int i = threadIdx.x; i < L; i += nthr) {
+                uint32_t ci = C[i];
+                if (ci) atomicAdd(&A.out[S[i]], 2u * ci);
+            }
+        }
+        if (threadIdx.x == 0 && hjob) atomicAdd(&A.out[J.owner], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
+        hits_all += (threadIdx.x == 0) ? hjob : 0ull;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && hits_all) atomicAdd(A.global, (ORIENTED ? 6ull : 1ull) * hits_all);
+    if (A.strat) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            unsigned long long s = wave_sum<unsigned long long>(sc[q]);
+            if (lane == 0 && s) atomicAdd(&A.strat[q], s);
+        }
+    }
+}
+
+// ------------------------------------------------------------- driver --
+
+tc_schedule default_schedule()
+{
+    tc_schedule s = {};
+    s.method = TC_METHOD_AUTO;
+    s.policy = TC_POLICY_FIXED;
+    s.small_max = 64;
+    s.mid_max = 512;
+    s.job_items = 8192;
+    s.ratio_x16 = 64;  // merge when long <= 4x short (and short lists)
+    s.persistent = 1;
+    return s;
+}
+
+namespace {
+
+constexpr int kL0 = 64, kL1 = 512, kL2 = 2048;
+
+int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const CountArgs &A,
+               hipStream_t s)
+{
+    if (njobs <= 0) return TC_OK;
+    int dev = 0;
+    TC_HIP(hipGetDevice(&dev));
+    int cus = 0;
+    TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // persistent grid: enough resident blocks to fill every CU
+    int per_cu = bin == 0 ? 16 : (bin == 1 ? 8 : 4);
+    int64_t grid = std::min<int64_t>(njobs, (int64_t)cus * per_cu);
+    if (bin == 0) {
+        if (oriented)
+            hipLaunchKernelGGL((k_count<kL0, 1, true>), dim3((unsigned)grid), dim3(64), 0, s, jobs, njobs, A);
+        else
+            hipLaunchKernelGGL((k_count<kL0, 1, false>), dim3((unsigned)grid), dim3(64), 0, s, jobs, njobs, A);
+    } else if (bin == 1) {
+        if (oriented)
+            hipLaunchKernelGGL((k_count<kL1, 2, true>), dim3((unsigned)grid), dim3(128), 0, s, jobs, njobs, A);
+        else
+            hipLaunchKernelGGL((k_count<kL1, 2, false>), dim3((unsigned)grid), dim3(128), 0, s, jobs, njobs, A);
+    } else {
+        if (oriented)
+            hipLaunchKernelGGL((k_count<kL2, 4, true>), dim3((unsigned)grid), dim3(256), 0, s, jobs, njobs, A);
+        else
+            hipLaunchKernelGGL((k_count<kL2, 4, false>), dim3((unsigned)grid), dim3(256), 0, s, jobs, njobs, A);
+    }
+    TC_HIP(hipGetLastError());
+    return TC_OK;
+}
+
+}  // namespace
+
+int build_direct_vinfo(Graph &g)
+{
+    const int32_t nv = g.st.nv;
+    TC_TRY(g.vinfo.ensure((size_t)std::max<int32_t>(nv, 1) * sizeof(VInfo)));
+    hipLaunchKernelGGL(k_direct_vinfo, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, g.stream,
+                       g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), nv, g.vinfo.as<VInfo>());
+    TC_HIP(hipGetLastError());
+    return TC_OK;
+}
+
+extern const uint8_t *learned_table_device(int32_t *dim);
+
+int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &sched,
+                 tc_stats *stats, bool skip_lower)
+{
+    const int32_t nv = g.st.nv;
+    const int64_t nnz = g.st.nnz;
+    hipStream_t s = g.stream;
+    TC_TRY(g.ensure_events());
+    tc_stats st = {};
+
+    int method = sched.method;
+    if (skip_lower) method = TC_METHOD_DIRECT;
+    if (method == TC_METHOD_AUTO || method == TC_METHOD_ORIENTED) {
+        int64_t bad = 0;
+        if (g.symmetric < 0 || g.self_loops < 0) TC_TRY(validate_graph(g, true, &bad));
+        bool ok = g.symmetric == 1 && g.self_loops == 0;
+        if (method == TC_METHOD_ORIENTED && !ok) {
+            set_last_error(TC_ERR_INVALID,
+                           "oriented method needs a symmetric graph without self-loops");
+            return TC_ERR_INVALID;
+        }
+        method = ok ? TC_METHOD_ORIENTED : TC_METHOD_DIRECT;
+    }
+    const bool oriented = method == TC_METHOD_ORIENTED;
+    st.method_used = method;
+
+    // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin stats
+    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
+    unsigned long long *acc = g.accum.as<unsigned long long>();
+    uint32_t *outp = out;
+    DevBuf tmp_out;
+    if (!outp) {
+        TC_TRY(g.tmp4.ensure((size_t)std::max<int32_t>(nv, 1) * sizeof(uint32_t)));
+        outp = g.tmp4.as<uint32_t>();
+    }
+
+    TC_HIP(hipEventRecord(g.ev[0], s));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, acc, (int64_t)16);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, outp,
+                       (int64_t)nv, 0u);
+    TC_HIP(hipGetLastError());
+
+    if (nv == 0 || nnz == 0) {
+        TC_HIP(hipEventRecord(g.ev[1], s));
+        TC_HIP(hipEventSynchronize(g.ev[1]));
+        if (global) *global = 0;
+        if (stats) *stats = st;
+        return TC_OK;
+    }
+
+    // compact offsets of the store rows
+    TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
+    int64_t *coff = g.coff.as<int64_t>();
+    TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), coff, nv, g.tmp, s));
+
+    TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
+    VInfo *vi = g.vinfo.as<VInfo>();
+    const int32_t *adj = nullptr;
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
+    int32_t maxplus_h = 0;
+    if (oriented) {
+        TC_TRY(g.padj.ensure((size_t)nnz * sizeof(int32_t)));
+        // scratch: mask (u64 x nchunks), cnt (i32 x nchunks), cscan (i64 x nchunks+1)
+        size_t need = (size_t)nchunks * 8 + (size_t)nchunks * 4 + (size_t)(nchunks + 1) * 8 + 64;
+        TC_TRY(g.tmp2.ensure(need));
+        uint64_t *mask = g.tmp2.as<uint64_t>();
+        int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
+        int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
+        int32_t *maxplus = reinterpret_cast<int32_t *>(acc + 15);
+        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, coff,
+                           g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
+                           nv, nnz, mask, cnt);
+        TC_HIP(hipGetLastError());
+        TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
+        hipLaunchKernelGGL(k_orient_vinfo, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff,
+                           g.st.len.as<int32_t>(), cscan, mask, nv, vi, maxplus);
+        TC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_orient_write, dim3(wblocks), dim3(256), 0, s, coff,
+                           g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, vi, nv,
+                           nnz, g.padj.as<int32_t>());
+        TC_HIP(hipGetLastError());
+        adj = g.padj.as<int32_t>();
+    } else {
+        TC_TRY(build_direct_vinfo(g));
+        adj = g.st.pool.as<int32_t>();
+    }
+
+    // ---- planning: work per owner, jobs per bin
+    PlanParams P;
+    P.lmax[0] = kL0;
+    P.lmax[1] = kL1;
+    P.lmax[2] = kL2;
+    P.bin_max[0] = std::min(std::max(sched.small_max, 1), kL0);
+    P.bin_max[1] = std::min(std::max(sched.mid_max, P.bin_max[0]), kL1);
+    P.job_items = std::max(sched.job_items, 256);
+    TC_TRY(g.work.ensure((size_t)nv * sizeof(unsigned long long)));
+    unsigned long long *work = g.work.as<unsigned long long>();
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, work, (int64_t)nv);
+    if (oriented)
+        hipLaunchKernelGGL(k_owner_work<true>, dim3(wblocks), dim3(256), 0, s, coff, vi, adj, nv,
+                           nnz, P, work, acc + 4);
+    else
+        hipLaunchKernelGGL(k_owner_work<false>, dim3(wblocks), dim3(256), 0, s, coff, vi, adj, nv,
+                           nnz, P, work, acc + 4);
+    TC_HIP(hipGetLastError());
+    TC_TRY(g.tmp3.ensure((size_t)3 * nv * sizeof(int32_t) + (size_t)3 * (nv + 1) * sizeof(int64_t) + 16));
+    int32_t *njobs = g.tmp3.as<int32_t>();
+    int64_t *jscan = reinterpret_cast<int64_t *>(njobs + 3 * (int64_t)nv + ((3 * (int64_t)nv) & 1));
+    if (oriented)
+        hipLaunchKernelGGL(k_plan_jobs<true>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
+                           work, nv, P, njobs);
+    else
+        hipLaunchKernelGGL(k_plan_jobs<false>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
+                           work, nv, P, njobs);
+    TC_HIP(hipGetLastError());
+    for (int b = 0; b < 3; ++b)
+        TC_TRY(scan_counts_i32(njobs + (int64_t)b * nv, jscan + (int64_t)b * (nv + 1), nv, g.tmp, s));
+    int64_t jt[3];
+    for (int b = 0; b < 3; ++b)
+        TC_HIP(hipMemcpyAsync(&jt[b], jscan + (int64_t)b * (nv + 1) + nv, sizeof(int64_t),
+                              hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    TC_TRY(g.jobs.ensure((size_t)(jt[0] + jt[1] + jt[2] + 1) * sizeof(Job)));
+    Job *j0 = g.jobs.as<Job>();
+    Job *j1 = j0 + jt[0];
+    Job *j2 = j1 + jt[1];
+    if (oriented)
+        hipLaunchKernelGGL(k_emit_jobs<true>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
+                           njobs, jscan, nv, P, j0, j1, j2);
+    else
+        hipLaunchKernelGGL(k_emit_jobs<false>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
+                           njobs, jscan, nv, P, j0, j1, j2);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipEventRecord(g.ev[2], s));
+
+    // ---- count
+    CountArgs A;
+    A.adj = adj;
+    A.vi = vi;
+    A.out = outp;
+    A.global = acc;
+    A.strat = acc + 1;
+    A.policy = sched.policy;
+    A.ratio_x16 = sched.ratio_x16 > 0 ? sched.ratio_x16 : 64;
+    A.skip_lower = skip_lower ? 1 : 0;
+    A.table = nullptr;
+    A.table_dim = 0;
+    if (sched.policy == TC_POLICY_LEARNED) {
+        int32_t dim = 0;
+        A.table = learned_table_device(&dim);
+        A.table_dim = dim;
+    }
+    const Job *jl[3] = {j0, j1, j2};
+    for (int b = 0; b < 3; ++b) {
+        TC_HIP(hipEventRecord(g.ev[3 + b], s));
+        TC_TRY(launch_bin(b, oriented, jl[b], jt[b], A, s));
+    }
+    TC_HIP(hipEventRecord(g.ev[6], s));
+    TC_HIP(hipEventSynchronize(g.ev[6]));
+
+    unsigned long long hacc[16];
+    TC_HIP(hipMemcpy(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost));
+    if (global) *global = hacc[0];
+    if (stats) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, g.ev[0], g.ev[6]);
+        st.ms_total = ms;
+        hipEventElapsedTime(&ms, g.ev[0], g.ev[2]);
+        st.ms_prepare = ms;
+        hipEventElapsedTime(&ms, g.ev[3], g.ev[6]);
+        st.ms_count = ms;
+        for (int b = 0; b < 3; ++b) {
+            hipEventElapsedTime(&ms, g.ev[3 + b], g.ev[4 + b]);
+            st.ms_bin[b] = ms;
+            st.jobs_bin[b] = jt[b];
+            st.tasks_bin[b] = (int64_t)hacc[4 + 2 * b];
+            st.bytes_alg_bin[b] = (double)hacc[5 + 2 * b];
+        }
+        for (int q = 0; q < 3; ++q) st.strategy_tasks[q] = (int64_t)hacc[1 + q];
+        st.triangles = oriented ? hacc[0] / 6 : 0;
+        *stats = st;
+    }
+    (void)maxplus_h;
+    return TC_OK;
+}
+
+}  // namespace tc

@@ -1,0 +1,267 @@
+/*
+ * custinger_tc.h -- C-ABI of the MI355X triangle-counting engine.
+ *
+ * Drop-in boundary for cuStingerAlg's static and streaming triangle counting.
+ * Plain pointers and sizes only; every entry point returns an int status
+ * (TC_OK = 0) and never throws.  Device pointers are HIP device addresses on
+ * the current device; "host" arguments are ordinary host memory.
+ *
+ * Count convention (bit-exact with the reference):
+ *   per-vertex  uint32 out[v] = sum over neighbours u != v with deg(u) >= 2 and
+ *               deg(v) >= 2 of |N(v) cap N(u)|, modulo 2^32 -- for a simple
+ *               undirected graph this is 2*t(v)
+ *               (src/static_triangle_counting/cct.cu:233-268);
+ *   global      uint64 = the exact (unwrapped) sum of the per-vertex values
+ *               (6*T for a simple undirected graph; tests/cct.cpp:119-125 sums
+ *               the same values into int64).
+ *
+ * Threading: a handle is used by one host thread at a time; all work of a
+ * handle is ordered on its stream (tc_graph_set_stream).
+ */
+#ifndef CUSTINGER_TC_H
+#define CUSTINGER_TC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status -- */
+enum {
+    TC_OK = 0,
+    TC_ERR_INVALID = 1,   /* bad argument or malformed graph/batch        */
+    TC_ERR_OOM = 2,       /* device allocation failed                     */
+    TC_ERR_HIP = 3,       /* HIP runtime error (see tc_last_error)        */
+    TC_ERR_UNSUPPORTED = 4,
+    TC_ERR_CAPACITY = 5   /* input larger than the entry point's limit    */
+};
+
+/* Message of the last failing call on this thread (never NULL). */
+const char *tc_last_error(void);
+const char *tc_status_string(int status);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int tc_version(void);
+
+/* ------------------------------------------------ device memory utils -- */
+/* allocDeviceArray / freeDeviceArray / copyArray{HostToDevice,DeviceToHost,
+ * DeviceToDevice} of the cuStinger utilities (used at include/algs.cuh:41,73,101). */
+int tc_device_alloc(void **ptr, uint64_t bytes);
+int tc_device_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device, 3 host->host */
+int tc_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
+int tc_memset(void *dst, int value, uint64_t bytes);
+int tc_device_synchronize(void);
+int tc_device_name(char *buf, int len);
+
+/* ------------------------------------------------------- graph store -- */
+typedef struct tc_graph tc_graph;
+
+/* Location of CSR input arrays. */
+#define TC_HOST 0
+#define TC_DEVICE 1
+
+/* Create flags. */
+#define TC_GRAPH_ASSUME_SYMMETRIC 0x1u /* skip the symmetry check (caller vouches) */
+#define TC_GRAPH_NO_VALIDATE      0x2u /* skip sortedness/range validation        */
+
+/* cuStinger::initializeCuStinger with eInitStateCSR (tests/cct.cpp:261-277):
+ * build the dynamic per-vertex block store from a CSR (off[nv+1] int64,
+ * adj[nnz] int32; every row sorted strictly ascending, ids in [0,nv)).
+ * slack_pct adds capacity headroom per block for later insertions
+ * (0 = exact fit; blocks still round up to a power of two when they grow). */
+int tc_graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t *adj,
+                        int location, uint32_t flags, int32_t slack_pct, tc_graph **out);
+/* cuStinger::freecuStinger (tests/cct.cpp:281). */
+int tc_graph_destroy(tc_graph *g);
+/* Bind the handle to a HIP stream (NULL = the default stream). */
+int tc_graph_set_stream(tc_graph *g, void *hip_stream);
+
+typedef struct {
+    int32_t nv;
+    int64_t nnz;               /* entries used (getNumberEdgesUsed, tests/cct_upd.cpp:127) */
+    int64_t allocated;         /* entries allocated (getNumberEdgesAllocated, :128)       */
+    int32_t symmetric;         /* 1 if every (u,v) has its (v,u)                         */
+    int64_t self_loops;        /* number of (v,v) entries                                 */
+    int32_t max_degree;
+    int64_t reallocations;     /* block moves since creation ("allocs", :392,398)         */
+} tc_graph_info;
+int tc_graph_get_info(tc_graph *g, tc_graph_info *info);
+
+/* Snapshot of the store as a compact CSR (location TC_HOST or TC_DEVICE);
+ * off has nv+1 entries, adj room for info.nnz entries. */
+int tc_graph_to_csr(tc_graph *g, int64_t *off, int32_t *adj, int location);
+/* Per-vertex degrees (dVD->getUsed(), cct.cu:235) into a device int32[nv]. */
+int tc_graph_degrees(tc_graph *g, int32_t *deg_dev);
+
+/* ---------------------------------------------------------- schedule -- */
+/* Strategy policy for the per-edge intersection. */
+#define TC_POLICY_FIXED   0   /* degree-ratio heuristic                    */
+#define TC_POLICY_LEARNED 1   /* MLP cost model table (tc_model_*)         */
+#define TC_POLICY_MERGE   2   /* force one strategy (tests, timing)        */
+#define TC_POLICY_BSEARCH 3
+#define TC_POLICY_HASH    4
+
+/* Counting method. */
+#define TC_METHOD_AUTO     0  /* oriented if symmetric and loop-free, else direct */
+#define TC_METHOD_ORIENTED 1  /* degree-oriented, each triangle found once         */
+#define TC_METHOD_DIRECT   2  /* every directed entry, like cct.cu:238-266         */
+
+typedef struct {
+    int32_t method;        /* TC_METHOD_*                                           */
+    int32_t policy;        /* TC_POLICY_*                                           */
+    int32_t small_max;     /* rows with |N+| <= small_max use one wave per job      */
+    int32_t mid_max;       /* rows with |N+| <= mid_max use the mid bin              */
+    int32_t job_items;     /* target items per job (prefix-sum partition grain)      */
+    int32_t ratio_x16;     /* fixed heuristic: bsearch when short*log2(long)*16 <    */
+                           /*   long*ratio_x16                                        */
+    int32_t persistent;    /* 1: persistent job-queue launch                          */
+    int32_t reserved[9];
+} tc_schedule;
+
+/* Fill *s with the defaults (autotuned table if one was loaded). */
+int tc_schedule_default(tc_schedule *s);
+
+typedef struct {
+    double ms_total;        /* hipEvent time of the whole count               */
+    double ms_prepare;      /* orientation + binning                          */
+    double ms_count;        /* intersection kernels                           */
+    double ms_bin[4];       /* per bin (small, mid, large, direct)            */
+    double bytes_alg_bin[4];/* algorithmic bytes 4*(L_u+L_v) per bin          */
+    int64_t tasks_bin[4];   /* intersection tasks per bin                     */
+    int64_t jobs_bin[4];
+    int64_t strategy_tasks[3]; /* tasks run as merge / bsearch / hash         */
+    uint64_t triangles;     /* T for simple graphs (global/6), else 0         */
+    int32_t method_used;
+    int32_t reserved[7];
+} tc_stats;
+
+/* ------------------------------------------------------ static count -- */
+/* callDeviceAllTriangles (include/static_triangle_counting/cct.hpp:15-17,
+ * src/static_triangle_counting/cct.cu:272-277): per_vertex_dev (device
+ * uint32[nv], may be NULL) is overwritten with the per-vertex counts;
+ * global_host (may be NULL) receives the uint64 total.  sched and stats may
+ * be NULL.  Synchronous with respect to the handle's stream. */
+int tc_static_count(tc_graph *g, uint32_t *per_vertex_dev, uint64_t *global_host,
+                    const tc_schedule *sched, tc_stats *stats);
+
+/* callDeviceAllTrianglesCSR (tests/cct_csr.cu:279-285): counts on a raw
+ * device CSR with the CSR kernel's convention (entries with dest < src are
+ * skipped, sum = 3T).  d_off int64[nv+1], d_ind int32[nnz], out uint32[nv]. */
+int tc_csr_count_device(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_t *d_ind,
+                        uint32_t *out_dev, uint64_t *global_host, void *hip_stream);
+
+/* ------------------------------------------------------ batch update -- */
+typedef struct {
+    int64_t inserted;        /* undirected pairs newly inserted                  */
+    int64_t deleted;         /* undirected pairs removed                          */
+    int64_t dup_inserts;     /* no-op inserts (in-batch repeat or present)        */
+    int64_t missing_deletes; /* no-op deletes (absent pair or in-batch repeat)    */
+    int64_t reallocations;   /* blocks moved to grow                              */
+    double ms_update;        /* store update time                                 */
+    double ms_delta;         /* delta counting time                               */
+    int32_t reserved[8];
+} tc_batch_stats;
+
+/* Streaming batch: cuStinger edgeInsertionsSorted / edgeDeletions
+ * (tests/cct_upd.cpp:395-398, tests/katz-tester.cu:244) fused with
+ * callDeviceNewTriangles (src/streaming_triangle_counting/cct_upd.cu:412-440).
+ * u, v: B undirected pairs (device int32); is_delete: device uint8[B] or NULL
+ * (all insertions).  Deletions are counted on the pre-deletion graph and
+ * removed first, then insertions are applied and counted.  Self-loops and
+ * out-of-range ids are rejected (TC_ERR_INVALID, store untouched).
+ * per_vertex_delta_dev (device uint32[nv], may be NULL) ACCUMULATES the
+ * per-vertex delta (mod 2^32), like the reference's outPutTriangles;
+ * delta_global_host (may be NULL) receives the signed change of the global
+ * count (6 * change in T). */
+int tc_batch_apply(tc_graph *g, const int32_t *u, const int32_t *v, const uint8_t *is_delete,
+                   int64_t B, uint32_t *per_vertex_delta_dev, int64_t *delta_global_host,
+                   const tc_schedule *sched, tc_batch_stats *stats);
+
+/* Store-only update (no counting): cuStinger edgeInsertions / edgeDeletions. */
+int tc_batch_update_only(tc_graph *g, const int32_t *u, const int32_t *v,
+                         const uint8_t *is_delete, int64_t B, tc_batch_stats *stats);
+
+/* Reference-faithful delta kernels on a prepared batch, for parity of the
+ * three passes (cct_upd.cu:231-410): s_src/s_dst sorted directed entries,
+ * dup flags (device int32[nb]), boff int64[nv+1] batch offsets; the graph
+ * must already contain the batch.  pass_mask bit0/1/2 selects the passes;
+ * out_dev accumulates. */
+int tc_stream_delta_passes(tc_graph *g, int64_t nb, const int32_t *s_src, const int32_t *s_dst,
+                           const int32_t *dup, const int64_t *boff, uint32_t *out_dev,
+                           int32_t pass_mask);
+
+/* ---------------------------------------------------- batched graphs -- */
+/* Batched static TC over G independent graphs concatenated into one CSR with
+ * global vertex ids: graph i owns vertices [voff[i], voff[i+1]); its entries
+ * must stay inside that range.  off int64[voff[G]+1], adj int32, all device
+ * pointers.  per_graph_dev: device uint64[G] (global count per graph);
+ * per_vertex_dev: device uint32[voff[G]] or NULL. */
+int tc_batched_count(int32_t G, const int64_t *voff_dev, const int64_t *off_dev,
+                     const int32_t *adj_dev, int64_t nnz, int32_t nv_total,
+                     uint64_t *per_graph_dev, uint32_t *per_vertex_dev,
+                     const tc_schedule *sched, tc_stats *stats, void *hip_stream);
+
+/* ----------------------------------------------------- load balancing -- */
+/* cusLoadBalance::LoadBalanceArray (include/load_balance.cuh:77-102):
+ * work[i] = degree of verts[i] (device int32 ids, len entries); returns the
+ * partition count in *P and the lower-bound partition points in parts_dev
+ * (device int64, room for 1000). */
+int tc_lb_partition(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t *parts_dev,
+                    int64_t *P);
+/* Degree binning of the LB traversal (include/operators.cuh:164-288):
+ * counts of vertices with degree >= 64, in [16,64) and < 16 among verts. */
+int tc_lb_degree_bins(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t counts[3]);
+
+/* ---------------------------------------------------------- generators -- */
+/* Counter-based R-MAT / uniform pair generators and CSR canonicaliser
+ * (symmetrise, drop self-loops, dedup, sort) on the device.  The same
+ * generator is restated on the CPU so a graph can be rebuilt anywhere. */
+int tc_gen_rmat_pairs(int32_t scale, int64_t m, double a, double b, double c, uint64_t seed,
+                      int32_t *u_dev, int32_t *v_dev, void *hip_stream);
+int tc_gen_er_pairs(int32_t n, int64_t m, uint64_t seed, int32_t *u_dev, int32_t *v_dev,
+                    void *hip_stream);
+/* Builds a canonical CSR from m device pairs; *off_dev (int64[nv+1]) and
+ * *adj_dev are allocated by the call (free with tc_device_free). */
+int tc_canon_csr_device(int32_t nv, int64_t m, const int32_t *u_dev, const int32_t *v_dev,
+                        int64_t **off_dev, int32_t **adj_dev, int64_t *nnz, void *hip_stream);
+
+/* ------------------------------------------------ learned cost model -- */
+/* 3-layer MLP (4 -> 32 -> 32 -> 3) over (log2 |short|, log2 |long|,
+ * log2 ratio, bin) predicting per-strategy cost; trained on the GPU. */
+#define TC_MLP_IN 4
+#define TC_MLP_HIDDEN 32
+#define TC_MLP_OUT 3
+#define TC_MLP_PARAMS (TC_MLP_IN * TC_MLP_HIDDEN + TC_MLP_HIDDEN + \
+                       TC_MLP_HIDDEN * TC_MLP_HIDDEN + TC_MLP_HIDDEN + \
+                       TC_MLP_HIDDEN * TC_MLP_OUT + TC_MLP_OUT)
+
+/* Timing collection: runs every strategy on a sample of up to max_samples
+ * intersection tasks of the graph and writes per-sample features
+ * (float[4]) and measured cycles per strategy (float[3]) to host arrays.
+ * Returns the number of samples in *n. */
+int tc_collect_timings(tc_graph *g, int64_t max_samples, uint64_t seed, float *feat_host,
+                       float *cost_host, int64_t *n);
+
+/* One FP32 training step on device: forward, MSE loss on log-cost,
+ * backward, SGD/Adam update of params_dev (float[TC_MLP_PARAMS]).
+ * grads_dev (may be NULL) receives the gradient; loss_host the loss. */
+int tc_mlp_train_step(float *params_dev, float *adam_m_dev, float *adam_v_dev, int32_t step,
+                      const float *feat_dev, const float *target_dev, int64_t n, float lr,
+                      float *grads_dev, float *loss_host);
+/* FP32 forward on device (predictions float[n*3]). */
+int tc_mlp_forward(const float *params_dev, const float *feat_dev, int64_t n, float *pred_dev);
+/* BF16 inference over the quantised degree grid: fills the strategy table
+ * used by TC_POLICY_LEARNED. */
+int tc_model_load(const float *params_host);
+int tc_model_table(uint8_t *table_host, int32_t *dim);
+
+/* -------------------------------------------------------- autotuning -- */
+/* Sweeps bin thresholds and job grain on g and stores the best schedule as
+ * the process default (the reference's exhaustive sweep, tests/cct.cpp:177-209). */
+int tc_autotune(tc_graph *g, tc_schedule *best, double *best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUSTINGER_TC_H */

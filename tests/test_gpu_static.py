@@ -1,0 +1,162 @@
+"""Static triangle counting on the GPU against the oracle (bit-exact)."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import custingeralg_amd as T
+import oracle as O
+from tests import graphs as GR
+
+pytestmark = pytest.mark.gpu
+
+POLICIES = [T.POLICY_FIXED, T.POLICY_MERGE, T.POLICY_BSEARCH, T.POLICY_HASH]
+
+
+def count(off, adj, **sched):
+    g = T.cuStinger(off, adj)
+    try:
+        pv, glob, st = T.static_count(g, T.default_schedule(**sched))
+    finally:
+        g.close()
+    return pv, glob, st
+
+
+@pytest.mark.parametrize("name", sorted(GR.closed_form_cases()))
+@pytest.mark.parametrize("policy", POLICIES)
+def test_closed_forms(name, policy):
+    off, adj, t = GR.closed_form_cases()[name]
+    pv, glob, st = count(off, adj, policy=policy)
+    assert np.array_equal(pv, (2 * t).astype(np.uint32)), name
+    assert glob == 2 * int(t.sum())
+    assert st["method_used"] == T.METHOD_ORIENTED
+
+
+@pytest.mark.parametrize("name", sorted(GR.fixture_cases()))
+@pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
+def test_fixtures_networkx(name, method):
+    off, adj, t = GR.fixture_cases()[name]
+    pv, glob, _ = count(off, adj, method=method)
+    assert np.array_equal(pv, (2 * t).astype(np.uint32))
+    assert glob == 2 * int(t.sum())
+    assert np.array_equal(pv, O.custinger_count(off, adj))
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("loops", [False, True])
+def test_asymmetric_and_loops_direct(seed, loops):
+    """Inputs the oriented method cannot take fall back to the reference formula."""
+    off, adj = GR.random_lists(90, 0.12, seed, loops=loops)
+    ref = GR.brute_custinger(off, adj)
+    assert np.array_equal(ref, O.custinger_count(off, adj))
+    for policy in POLICIES:
+        pv, glob, st = count(off, adj, policy=policy)
+        assert st["method_used"] == T.METHOD_DIRECT
+        assert np.array_equal(pv, ref)
+        assert glob == int(ref.astype(np.uint64).sum())
+
+
+def test_csr_variant_3t():
+    off, adj = O.rmat_graph(12, 8, 5)
+    pv, glob = T.csr_count_device(off, adj)
+    ref = O.csr_count(off, adj)
+    assert np.array_equal(pv, ref)
+    Tt, _ = O.forward_count(off, adj)
+    assert glob == 3 * Tt
+
+
+def test_er16_plain_config():
+    """n=2^16, d=16, seed 1: the committed per-vertex golden vector."""
+    z = np.load(os.path.join(GR.GOLDEN, "fixtures.npz"))
+    off, adj = O.er_graph(1 << 16, 16, 1)
+    for method in (T.METHOD_AUTO, T.METHOD_DIRECT):
+        pv, glob, _ = count(off, adj, method=method)
+        assert np.array_equal(pv, z["er16_pv"])
+        assert glob == 6 * json.load(open(os.path.join(GR.GOLDEN, "fixtures.json")))["er16"]["T"]
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_rmat_against_reference_formula(scale):
+    off, adj = O.rmat_graph(scale, 16, 2)
+    ref = O.custinger_count(off, adj)
+    for policy in POLICIES:
+        pv, glob, _ = count(off, adj, policy=policy)
+        assert np.array_equal(pv, ref)
+        assert glob == int(ref.astype(np.uint64).sum())
+
+
+def test_rmat18_schedules_identical():
+    off, adj = O.rmat_graph(18, 16, 2)
+    Tt, tv = O.forward_count(off, adj, per_vertex=True)
+    want = (2 * tv % (1 << 32)).astype(np.uint32)
+    variants = [dict(), dict(small_max=16, mid_max=128, job_items=1024),
+                dict(small_max=64, mid_max=512, job_items=65536), dict(policy=T.POLICY_HASH),
+                dict(policy=T.POLICY_BSEARCH), dict(policy=T.POLICY_MERGE)]
+    for v in variants:
+        pv, glob, _ = count(off, adj, **v)
+        assert np.array_equal(pv, want), v
+        assert glob == 6 * Tt
+
+
+def test_tiled_owner_lists():
+    # oriented: vertex 0 of K_2100 has |N+| = 2099 > the LDS tile
+    off, adj, t = GR.complete_fast(2100)
+    pv, glob, _ = count(off, adj)
+    assert np.all(pv == (2 * t).astype(np.uint32))
+    assert glob == 2 * int(t.sum())
+    # direct: the wheel hub has degree 3000
+    off, adj, t = GR.wheel(3000)
+    pv, glob, _ = count(off, adj, method=T.METHOD_DIRECT)
+    assert np.array_equal(pv, (2 * t).astype(np.uint32))
+
+
+def test_edge_cases():
+    for off, adj in [(np.zeros(1, np.int64), np.zeros(0, np.int32)),
+                     (np.zeros(6, np.int64), np.zeros(0, np.int32)),
+                     O.csr_from_edges(2, [(0, 1)]),
+                     O.csr_from_edges(10, [(0, 1), (1, 2), (0, 2)])]:
+        pv, glob, _ = count(off, adj)
+        assert np.array_equal(pv, O.custinger_count(off, adj))
+        assert glob == int(O.custinger_count(off, adj).astype(np.uint64).sum())
+
+
+def test_device_generators_match_oracle():
+    for scale, ef, seed in [(10, 8, 3), (16, 16, 2)]:
+        g = T.rmat_graph_device(scale, ef, seed)
+        off, adj = g.to_csr()
+        g.close()
+        roff, radj = O.rmat_graph(scale, ef, seed)
+        assert np.array_equal(off, roff) and np.array_equal(adj, radj)
+    g = T.er_graph_device(1 << 16, 16, 1)
+    off, adj = g.to_csr()
+    g.close()
+    roff, radj = O.er_graph(1 << 16, 16, 1)
+    assert np.array_equal(off, roff) and np.array_equal(adj, radj)
+
+
+def test_class_surface():
+    off, adj, t = GR.wheel(64)
+    g = T.cuStinger(off, adj)
+    alg = T.StaticTriangleCounting()
+    alg.Init(g)
+    alg.Run(g)
+    alg.SyncHostWithDevice()
+    assert np.array_equal(alg.host_out, (2 * t).astype(np.uint32))
+    assert alg.getGlobal() == 2 * int(t.sum())
+    alg.Reset()
+    assert not alg.copyPerVertexToHost().any()
+    alg.Release()
+    g.freecuStinger()
+
+
+def test_validation_rejects_malformed():
+    off = np.array([0, 2, 3], np.int64)
+    adj = np.array([1, 1, 0], np.int32)  # duplicate in row 0
+    with pytest.raises(T.TCError):
+        T.cuStinger(off, adj)
+    adj = np.array([1, 7, 0], np.int32)  # out of range
+    with pytest.raises(T.TCError):
+        T.cuStinger(off, adj)
