@@ -369,7 +369,7 @@ void graph_destroy(Graph *g)
     if (!g) return;
     hipDeviceSynchronize();
     DevBuf *bufs[] = {&g->st.boff, &g->st.len, &g->st.cap, &g->st.pool, &g->vinfo, &g->padj,
-                      &g->coff, &g->work, &g->jobs, &g->tmp, &g->tmp2, &g->tmp3, &g->tmp4,
+                      &g->coff, &g->crow, &g->work, &g->jobs, &g->tmp, &g->tmp2, &g->tmp3, &g->tmp4,
                       &g->accum, &g->bsrc, &g->bdst, &g->bflag, &g->bkeys, &g->bkeys2,
                       &g->bvals2, &g->boffb, &g->bdel};
     for (DevBuf *b : bufs) b->release();

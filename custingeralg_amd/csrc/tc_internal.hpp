@@ -111,7 +111,7 @@ struct Graph {
     int64_t self_loops = -1;   // -1 unknown
     int32_t max_degree = 0;
     // cached derived structures (rebuilt after updates)
-    DevBuf vinfo, padj, coff, work, jobs, tmp, tmp2, tmp3, tmp4, accum;
+    DevBuf vinfo, padj, coff, crow, work, jobs, tmp, tmp2, tmp3, tmp4, accum;
     DevBuf bsrc, bdst, bflag, bkeys, bkeys2, bvals2, boffb, bdel;
     hipEvent_t ev[16] = {};
     bool events_ready = false;

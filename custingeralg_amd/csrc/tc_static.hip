@@ -30,6 +30,30 @@
 namespace tc {
 
 enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2 };
+constexpr int kUnroll = 4;
+constexpr int kBigTask = 64;  // hash tasks at least this long get the whole wave
+
+__device__ inline int64_t readlane_i64(int64_t x, int lane)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)(uint64_t)x, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)x >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Position of w in the LDS owner tile, or -1.  H holds (position << 16) |
+// (low 16 bits of the id) per slot, linear probing, all-ones when empty.
+__device__ inline int32_t hash_find(const uint32_t *H, const int32_t *S, int32_t w, int hb,
+                                    int32_t HS)
+{
+    const uint32_t fp = (uint32_t)w & 0xFFFFu;
+    uint32_t slot = lds_hash(w, hb);
+    while (true) {
+        const uint32_t h = H[slot];
+        if (h == 0xFFFFFFFFu) return -1;
+        if ((h & 0xFFFFu) == fp && S[h >> 16] == w) return (int32_t)(h >> 16);
+        slot = (slot + 1) & (HS - 1);
+    }
+}
 
 // ------------------------------------------------------------ helpers --
 
@@ -79,21 +103,32 @@ __device__ inline int32_t row_search(const int64_t *__restrict__ coff, int64_t e
     return lo;
 }
 
-// Rows of the 64 entries [e0, e0+64) of a wave (entries past nnz get the
-// last row; callers mask them).
-__device__ inline int32_t wave_row_of(const int64_t *__restrict__ coff, int32_t nv, int64_t e0,
+// Row of each of the 64 entries [64c, 64c+64) of a wave.  crow[c] is the row
+// holding entry 64c (k_chunk_rows); the rows of the chunk lie in
+// [crow[c], crow[c+1]].  Entries past nnz get the last row; callers mask them.
+__device__ inline int32_t wave_row_of(const int64_t *__restrict__ coff,
+                                      const int32_t *__restrict__ crow, int32_t nv, int64_t c,
                                       int64_t nnz)
 {
     const int lane = threadIdx.x & (kWave - 1);
+    const int64_t e0 = c * kWave;
     int64_t e = e0 + lane;
-    int64_t elast = min(e0 + kWave, nnz) - 1;
-    int32_t rfirst = 0, rlast = 0;
-    if (lane == 0) rfirst = row_search(coff, e0, 0, nv - 1);
-    if (lane == 1) rlast = row_search(coff, elast, 0, nv - 1);
-    rfirst = __shfl(rfirst, 0, kWave);
-    rlast = __shfl(rlast, 1, kWave);
+    const int64_t elast = min(e0 + kWave, nnz) - 1;
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int32_t rfirst = crow[c];
+    const int32_t rlast = (c + 1 < nchunks) ? crow[c + 1] : nv - 1;
     if (e > elast) e = elast;
     return row_search(coff, e, rfirst, rlast);
+}
+
+// crow[c] = row holding compact entry 64c.
+__global__ void k_chunk_rows(const int64_t *__restrict__ coff, int32_t nv, int32_t *__restrict__ crow)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = coff[r], b = coff[r + 1];
+        for (int64_t c = (a + kWave - 1) / kWave; c * kWave < b; ++c) crow[c] = (int32_t)r;
+    }
 }
 
 __device__ inline bool rank_less(int32_t a, int32_t la, int32_t b, int32_t lb)
@@ -118,7 +153,8 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n)
 // ---------------------------------------------------------- orientation --
 
 // mask[c] = bit i set iff compact entry 64c+i points to a higher-rank vertex.
-__global__ void k_orient_mask(const int64_t *__restrict__ coff, const int64_t *__restrict__ boff,
+__global__ void k_orient_mask(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
+                              const int64_t *__restrict__ boff,
                               const int32_t *__restrict__ len, const int32_t *__restrict__ pool,
                               int32_t nv, int64_t nnz, uint64_t *__restrict__ mask,
                               int32_t *__restrict__ cnt)
@@ -129,7 +165,7 @@ __global__ void k_orient_mask(const int64_t *__restrict__ coff, const int64_t *_
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         int64_t e = c * kWave + lane;
-        int32_t a = wave_row_of(coff, nv, c * kWave, nnz);
+        int32_t a = wave_row_of(coff, crow, nv, c, nnz);
         bool plus = false;
         if (e < nnz) {
             int32_t b = pool[boff[a] + (e - coff[a])];
@@ -174,7 +210,8 @@ __global__ void k_orient_vinfo(const int64_t *__restrict__ coff, const int32_t *
     if ((threadIdx.x & 63) == 0) atomicMax(maxplus, mp);
 }
 
-__global__ void k_orient_write(const int64_t *__restrict__ coff, const int64_t *__restrict__ boff,
+__global__ void k_orient_write(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
+                               const int64_t *__restrict__ boff,
                                const int32_t *__restrict__ pool, const int64_t *__restrict__ cscan,
                                const uint64_t *__restrict__ mask, const VInfo *__restrict__ vi,
                                int32_t nv, int64_t nnz, int32_t *__restrict__ padj)
@@ -186,7 +223,7 @@ __global__ void k_orient_write(const int64_t *__restrict__ coff, const int64_t *
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         int64_t e = c * kWave + lane;
-        int32_t a = wave_row_of(coff, nv, c * kWave, nnz);
+        int32_t a = wave_row_of(coff, crow, nv, c, nnz);
         if (e >= nnz) continue;
         uint64_t m = mask[c];
         bool plus = (m >> lane) & 1ull;
@@ -217,6 +254,7 @@ __global__ void k_direct_vinfo(const int64_t *__restrict__ boff, const int32_t *
 
 struct PlanParams {
     int32_t lmax[3];     // owner-list capacity of each bin's kernel
+    int32_t nw[3];       // wavefronts per block of each bin's kernel
     int32_t bin_max[2];  // |owner| <= bin_max[0] -> bin 0, <= bin_max[1] -> bin 1, else 2
     int32_t job_items;   // target estimated items per job
 };
@@ -237,7 +275,8 @@ __device__ inline int owner_bin(int32_t L, const PlanParams &P)
 
 // Per-owner estimated work (sum over its tasks); per-bin stats.
 template <bool ORIENTED>
-__global__ void k_owner_work(const int64_t *__restrict__ coff, const VInfo *__restrict__ vi,
+__global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
+                             const VInfo *__restrict__ vi,
                              const int32_t *__restrict__ adj, int32_t nv, int64_t nnz,
                              PlanParams P, unsigned long long *__restrict__ work,
                              unsigned long long *__restrict__ bstats)
@@ -248,7 +287,7 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const VInfo *__re
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         int64_t e = c * kWave + lane;
-        int32_t b = wave_row_of(coff, nv, c * kWave, nnz);
+        int32_t b = wave_row_of(coff, crow, nv, c, nnz);
         unsigned long long w = 0, bytes = 0;
         int bin = 0;
         bool task = false;
@@ -310,7 +349,9 @@ __global__ void k_plan_jobs(const VInfo *__restrict__ vi, const unsigned long lo
             int bin = owner_bin(L, P);
             int64_t tiles = ceil_div(L, P.lmax[bin]);
             int64_t per_tile = (int64_t)(w / (unsigned long long)tiles) + 1;
-            int64_t split = ceil_div(per_tile, P.job_items);
+            // a job amortises building its LDS table over at least 16 items per slot
+            int64_t grain = max<int64_t>((int64_t)P.job_items * P.nw[bin], 16 * (int64_t)min(L, P.lmax[bin]));
+            int64_t split = ceil_div(per_tile, grain);
             if (split > T) split = T;
             if (split < 1) split = 1;
             nj[bin] = (int32_t)(split * tiles);
@@ -403,8 +444,11 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
 {
     constexpr int HMAX = 2 * LMAX;
     constexpr int HBMAX = (HMAX <= 1) ? 1 : 32 - __builtin_clz(HMAX - 1);
+    static_assert(LMAX < 65535, "slot positions are 16-bit");
+    // S: owner-list tile (sorted); H: hash slots holding (position << 16 | low 16 bits of id),
+    // 0xFFFFFFFF when empty; C: per-position hit counters (third-vertex counts).
     __shared__ int32_t S[LMAX];
-    __shared__ int32_t H[HMAX];
+    __shared__ uint32_t H[HMAX];
     __shared__ uint32_t C[ORIENTED ? LMAX : 1];
     __shared__ int32_t pre[NW][kWave + 1];
     __shared__ int64_t tpb[NW][kWave];
@@ -430,20 +474,23 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         if (hb > HBMAX) hb = HBMAX;
         const int32_t HS = 1 << hb;
         for (int i = threadIdx.x; i < L; i += nthr) S[i] = A.adj[obase + s0 + i];
-        for (int i = threadIdx.x; i < HS; i += nthr) H[i] = -1;
+        for (int i = threadIdx.x; i < HS; i += nthr) H[i] = 0xFFFFFFFFu;
         if (ORIENTED)
             for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
         __syncthreads();
         for (int i = threadIdx.x; i < L; i += nthr) {
-            uint32_t slot = lds_hash(S[i], hb);
-            while (atomicCAS(&H[slot], -1, i) != -1) slot = (slot + 1) & (HS - 1);
+            const int32_t w = S[i];
+            const uint32_t tag = ((uint32_t)i << 16) | ((uint32_t)w & 0xFFFFu);
+            uint32_t slot = lds_hash(w, hb);
+            while (atomicCAS(&H[slot], 0xFFFFFFFFu, tag) != 0xFFFFFFFFu) slot = (slot + 1) & (HS - 1);
         }
         __syncthreads();
         const int32_t lo_v = S[0], hi_v = S[L - 1];
         unsigned long long hits_job = 0;
 
-        for (int32_t t0 = J.t_begin + wid * kWave; t0 < J.t_end; t0 += NW * kWave) {
-            const int32_t t = t0 + lane;
+        // tasks are dealt to the waves round-robin so short jobs still keep every wave busy
+        for (int32_t g0 = 0; J.t_begin + (g0 * kWave) * NW + wid < J.t_end; ++g0) {
+            const int32_t t = J.t_begin + (g0 * kWave + lane) * NW + wid;
             bool valid = t < J.t_end;
             int32_t tv = valid ? A.adj[vo.rb + t] : 0;
             VInfo vt = {0, 0, 0};
@@ -455,7 +502,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                 valid = false;
             if (!valid) pl = 0;
             if (tiled && pl > 0) {
-                // keep only the part of the task list inside the owner tile's range
+                // keep only the part of the task list inside the owner tile's id range
                 int32_t lo = 0, hi = pl;
                 while (lo < hi) {
                     int32_t mid = (lo + hi) >> 1;
@@ -478,71 +525,144 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             uint32_t mcount = 0;
             wave_sync();
 
-            // -- hash: stream the task list through the LDS hash of the owner list
-            {
-                int32_t items = (strat == S_HASH) ? pl : 0;
-                int32_t incl = wave_incl_scan_i32(items);
-                pre[wid][lane + 1] = incl;
-                if (lane == 0) pre[wid][0] = 0;
-                const int32_t total = __shfl(incl, kWave - 1, kWave);
-                wave_sync();
-                for (int32_t it = lane; it < total; it += kWave) {
-                    int32_t lo = 0, hi = kWave;
-                    while (hi - lo > 1) {
-                        int32_t mid = (lo + hi) >> 1;
-                        if (pre[wid][mid] <= it) lo = mid; else hi = mid;
+            // Long hash tasks run one at a time across the whole wave: the list
+            // base is wave-uniform (scalar registers), each lane streams every
+            // 64th entry through the LDS hash, and hits stay in registers.
+            uint64_t big = __ballot(strat == S_HASH && pl >= kBigTask);
+            while (big) {
+                const int jb = __ffsll((unsigned long long)big) - 1;
+                big &= big - 1;
+                const int32_t *lst = A.adj + readlane_i64(pb, jb);
+                const int32_t n = __builtin_amdgcn_readlane(pl, jb);
+                uint32_t h = 0;
+                for (int32_t k0 = lane; k0 < n; k0 += kUnroll * kWave) {
+                    int32_t w[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const int32_t k = k0 + u * kWave;
+                        w[u] = (k < n) ? lst[k] : -1;
                     }
-                    const int32_t j = lo;
-                    const int32_t k = it - pre[wid][j];
-                    const int32_t w = A.adj[tpb[wid][j] + k];
-                    uint32_t slot = lds_hash(w, hb);
-                    int32_t pos;
-                    while (true) {
-                        pos = H[slot];
-                        if (pos < 0 || S[pos] == w) break;
-                        slot = (slot + 1) & (HS - 1);
-                    }
-                    if (pos >= 0) {
-                        atomicAdd(&tcnt[wid][j], 1u);
-                        if (ORIENTED) atomicAdd(&C[pos], 1u);
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        if (w[u] < 0) continue;
+                        const int32_t pos = hash_find(H, S, w[u], hb, HS);
+                        if (pos >= 0) {
+                            ++h;
+                            if (ORIENTED) atomicAdd(&C[pos], 1u);
+                        }
                     }
                 }
+                h = wave_sum<uint32_t>(h);
+                if (lane == jb) tcnt[wid][jb] = h;
             }
-            // -- bsearch: each owner-list element searched in the task list
-            {
-                int32_t items = (strat == S_BSEARCH) ? L : 0;
-                int32_t incl = wave_incl_scan_i32(items);
+
+            // Two flattened phases over the 64 tasks of this wave: items of the
+            // hash phase are task-list entries, items of the search phase are
+            // owner-list entries.  Each lane keeps a running task index and a
+            // private hit count, flushed when its task changes, and works on
+            // kUnroll items at once so that their loads overlap.
+#pragma unroll 1
+            for (int phase = 0; phase < 2; ++phase) {
+                const int want = phase == 0 ? S_HASH : S_BSEARCH;
+                const int32_t items =
+                    (strat == want && !(phase == 0 && pl >= kBigTask)) ? (phase == 0 ? pl : L) : 0;
+                const int32_t incl = wave_incl_scan_i32(items);
                 wave_sync();
                 pre[wid][lane + 1] = incl;
                 if (lane == 0) pre[wid][0] = 0;
                 const int32_t total = __shfl(incl, kWave - 1, kWave);
                 wave_sync();
-                for (int32_t it = lane; it < total; it += kWave) {
-                    int32_t lo = 0, hi = kWave;
-                    while (hi - lo > 1) {
-                        int32_t mid = (lo + hi) >> 1;
-                        if (pre[wid][mid] <= it) lo = mid; else hi = mid;
+                if (total == 0) continue;
+                int32_t j = 0, myj = -1;
+                uint32_t myhits = 0;
+                for (int32_t base = lane; base < total; base += kUnroll * kWave) {
+                    int32_t jj[kUnroll], ww[kUnroll], kk[kUnroll];
+                    bool ok[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const int32_t it = base + u * kWave;
+                        ok[u] = it < total;
+                        jj[u] = j;
+                        kk[u] = 0;
+                        ww[u] = 0;
+                        if (ok[u]) {
+                            if (pre[wid][j + 1] <= it) {
+                                int32_t lo = j + 1, hi = kWave;  // largest j with pre[j] <= it
+                                while (hi - lo > 1) {
+                                    int32_t mid = (lo + hi) >> 1;
+                                    if (pre[wid][mid] <= it) lo = mid; else hi = mid;
+                                }
+                                j = lo;
+                            }
+                            jj[u] = j;
+                            kk[u] = it - pre[wid][j];
+                            if (phase == 0) ww[u] = A.adj[tpb[wid][j] + kk[u]];
+                        }
                     }
-                    const int32_t j = lo;
-                    const int32_t k = it - pre[wid][j];
-                    const int32_t w = S[k];
-                    const int32_t *lst = A.adj + tpb[wid][j];
-                    int32_t a = 0, b = tpl[wid][j];
-                    while (a < b) {
-                        int32_t mid = (a + b) >> 1;
-                        if (lst[mid] < w) a = mid + 1; else b = mid;
-                    }
-                    if (a < tpl[wid][j] && lst[a] == w) {
-                        atomicAdd(&tcnt[wid][j], 1u);
-                        if (ORIENTED) atomicAdd(&C[k], 1u);
+                    if (phase == 0) {
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            if (!ok[u]) continue;
+                            if (jj[u] != myj) {
+                                if (myhits) atomicAdd(&tcnt[wid][myj], myhits);
+                                myj = jj[u];
+                                myhits = 0;
+                            }
+                            const int32_t pos = hash_find(H, S, ww[u], hb, HS);
+                            if (pos >= 0) {
+                                ++myhits;
+                                if (ORIENTED) atomicAdd(&C[pos], 1u);
+                            }
+                        }
+                    } else {
+                        // interleaved lower_bound searches of owner entries in task lists
+                        const int32_t *lst[kUnroll];
+                        int32_t key[kUnroll], b0[kUnroll], n0[kUnroll], nn[kUnroll];
+                        int32_t steps = 0;
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            lst[u] = A.adj + tpb[wid][jj[u]];
+                            key[u] = S[kk[u]];
+                            n0[u] = ok[u] ? tpl[wid][jj[u]] : 0;
+                            nn[u] = n0[u];
+                            b0[u] = 0;
+                            steps = max(steps, 32 - __clz(n0[u]));
+                        }
+                        for (int32_t s = 0; s < steps; ++s) {
+#pragma unroll
+                            for (int u = 0; u < kUnroll; ++u) {
+                                if (nn[u] > 0) {
+                                    const int32_t half = nn[u] >> 1;
+                                    if (lst[u][b0[u] + half] < key[u]) {
+                                        b0[u] += half + 1;
+                                        nn[u] -= half + 1;
+                                    } else {
+                                        nn[u] = half;
+                                    }
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            if (!ok[u]) continue;
+                            if (jj[u] != myj) {
+                                if (myhits) atomicAdd(&tcnt[wid][myj], myhits);
+                                myj = jj[u];
+                                myhits = 0;
+                            }
+                            if (b0[u] < n0[u] && lst[u][b0[u]] == key[u]) {
+                                ++myhits;
+                                if (ORIENTED) atomicAdd(&C[kk[u]], 1u);
+                            }
+                        }
                     }
                 }
+                if (myhits) atomicAdd(&tcnt[wid][myj], myhits);
             }
             // -- merge: one lane walks both sorted lists
             if (strat == S_MERGE) {
                 const int32_t *lst = A.adj + pb;
                 int32_t i = 0, k = 0;
-                // skip owner entries below the task list's first element
                 {
                     int32_t first = lst[0], a = 0, b = L;
                     while (a < b) {
@@ -581,7 +701,8 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                 if (ci) atomicAdd(&A.out[S[i]], 2u * ci);
             }
         }
-        if (threadIdx.x == 0 && hjob) atomicAdd(&A.out[J.owner], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
+        if (threadIdx.x == 0 && hjob)
+            atomicAdd(&A.out[J.owner], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
         hits_all += (threadIdx.x == 0) ? hjob : 0ull;
         __syncthreads();
     }
@@ -612,7 +733,8 @@ tc_schedule default_schedule()
 
 namespace {
 
-constexpr int kL0 = 64, kL1 = 512, kL2 = 2048;
+constexpr int kL0 = 64, kL1 = 512, kL2 = 4096;
+constexpr int kNW0 = 1, kNW1 = 2, kNW2 = 8;
 
 int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const CountArgs &A,
                hipStream_t s)
@@ -623,23 +745,23 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
     int cus = 0;
     TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     // persistent grid: enough resident blocks to fill every CU
-    int per_cu = bin == 0 ? 16 : (bin == 1 ? 8 : 4);
+    int per_cu = bin == 0 ? 16 : (bin == 1 ? 8 : 2);
     int64_t grid = std::min<int64_t>(njobs, (int64_t)cus * per_cu);
     if (bin == 0) {
         if (oriented)
-            hipLaunchKernelGGL((k_count<kL0, 1, true>), dim3((unsigned)grid), dim3(64), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count<kL0, kNW0, true>), dim3((unsigned)grid), dim3(kNW0 * kWave), 0, s, jobs, njobs, A);
         else
-            hipLaunchKernelGGL((k_count<kL0, 1, false>), dim3((unsigned)grid), dim3(64), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count<kL0, kNW0, false>), dim3((unsigned)grid), dim3(kNW0 * kWave), 0, s, jobs, njobs, A);
     } else if (bin == 1) {
         if (oriented)
-            hipLaunchKernelGGL((k_count<kL1, 2, true>), dim3((unsigned)grid), dim3(128), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count<kL1, kNW1, true>), dim3((unsigned)grid), dim3(kNW1 * kWave), 0, s, jobs, njobs, A);
         else
-            hipLaunchKernelGGL((k_count<kL1, 2, false>), dim3((unsigned)grid), dim3(128), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count<kL1, kNW1, false>), dim3((unsigned)grid), dim3(kNW1 * kWave), 0, s, jobs, njobs, A);
     } else {
         if (oriented)
-            hipLaunchKernelGGL((k_count<kL2, 4, true>), dim3((unsigned)grid), dim3(256), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
         else
-            hipLaunchKernelGGL((k_count<kL2, 4, false>), dim3((unsigned)grid), dim3(256), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count<kL2, kNW2, false>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
     }
     TC_HIP(hipGetLastError());
     return TC_OK;
@@ -688,7 +810,6 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
     uint32_t *outp = out;
-    DevBuf tmp_out;
     if (!outp) {
         TC_TRY(g.tmp4.ensure((size_t)std::max<int32_t>(nv, 1) * sizeof(uint32_t)));
         outp = g.tmp4.as<uint32_t>();
@@ -717,6 +838,9 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     VInfo *vi = g.vinfo.as<VInfo>();
     const int32_t *adj = nullptr;
     const int64_t nchunks = ceil_div(nnz, kWave);
+    TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
+    int32_t *crow = g.crow.as<int32_t>();
+    hipLaunchKernelGGL(k_chunk_rows, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff, nv, crow);
     const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
     int32_t maxplus_h = 0;
     if (oriented) {
@@ -728,7 +852,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
         int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
         int32_t *maxplus = reinterpret_cast<int32_t *>(acc + 15);
-        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, coff,
+        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, coff, crow,
                            g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
                            nv, nnz, mask, cnt);
         TC_HIP(hipGetLastError());
@@ -736,7 +860,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         hipLaunchKernelGGL(k_orient_vinfo, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff,
                            g.st.len.as<int32_t>(), cscan, mask, nv, vi, maxplus);
         TC_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_orient_write, dim3(wblocks), dim3(256), 0, s, coff,
+        hipLaunchKernelGGL(k_orient_write, dim3(wblocks), dim3(256), 0, s, coff, crow,
                            g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, vi, nv,
                            nnz, g.padj.as<int32_t>());
         TC_HIP(hipGetLastError());
@@ -751,6 +875,9 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     P.lmax[0] = kL0;
     P.lmax[1] = kL1;
     P.lmax[2] = kL2;
+    P.nw[0] = kNW0;
+    P.nw[1] = kNW1;
+    P.nw[2] = kNW2;
     P.bin_max[0] = std::min(std::max(sched.small_max, 1), kL0);
     P.bin_max[1] = std::min(std::max(sched.mid_max, P.bin_max[0]), kL1);
     P.job_items = std::max(sched.job_items, 256);
@@ -758,11 +885,11 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     unsigned long long *work = g.work.as<unsigned long long>();
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, work, (int64_t)nv);
     if (oriented)
-        hipLaunchKernelGGL(k_owner_work<true>, dim3(wblocks), dim3(256), 0, s, coff, vi, adj, nv,
-                           nnz, P, work, acc + 4);
+        hipLaunchKernelGGL(k_owner_work<true>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj, nv,
+                           nnz, P, work, stats ? acc + 4 : nullptr);
     else
-        hipLaunchKernelGGL(k_owner_work<false>, dim3(wblocks), dim3(256), 0, s, coff, vi, adj, nv,
-                           nnz, P, work, acc + 4);
+        hipLaunchKernelGGL(k_owner_work<false>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj, nv,
+                           nnz, P, work, stats ? acc + 4 : nullptr);
     TC_HIP(hipGetLastError());
     TC_TRY(g.tmp3.ensure((size_t)3 * nv * sizeof(int32_t) + (size_t)3 * (nv + 1) * sizeof(int64_t) + 16));
     int32_t *njobs = g.tmp3.as<int32_t>();

@@ -48,6 +48,8 @@ def lib():
         L.tco_host_count_triangles.argtypes = [C.c_int32, i64p, i32p, C.c_void_p, C.c_int]
         L.tco_host_count_rows.restype = C.c_int64
         L.tco_host_count_rows.argtypes = [C.c_int32, i64p, i32p, C.c_int32, C.c_int32, C.c_int]
+        L.tco_host_count_entries.restype = C.c_int64
+        L.tco_host_count_entries.argtypes = [C.c_int32, i64p, i32p, i64p, C.c_int64, C.c_int]
         L.tco_custinger_count.argtypes = [C.c_int32, i64p, i32p, u32p, C.c_int]
         L.tco_csr_count.argtypes = [C.c_int32, i64p, i32p, u32p, C.c_int]
         L.tco_sum_triangle_array.restype = C.c_int64
@@ -148,6 +150,13 @@ def host_count_triangles(off, adj, per_edge=False, threads=1):
 
 def host_count_rows(off, adj, v0, v1, threads=1) -> int:
     return int(lib().tco_host_count_rows(len(off) - 1, off, _nz(adj), v0, v1, threads))
+
+
+def host_count_entries(off, adj, entries, threads=1) -> int:
+    """hostCountTriangles' per-entry intersections on a sample of entry indices."""
+    entries = np.ascontiguousarray(entries, np.int64)
+    return int(lib().tco_host_count_entries(len(off) - 1, off, _nz(adj), _nz(entries),
+                                            len(entries), threads))
 
 
 def custinger_count(off, adj, threads=None) -> np.ndarray:

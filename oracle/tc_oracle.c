@@ -58,6 +58,7 @@ void tco_rmat_thresholds(double a, double b, double c, uint64_t t[3])
 void tco_gen_rmat_pairs(int scale, int64_t m, const uint64_t t[3], uint64_t seed,
                         int32_t *u, int32_t *v)
 {
+#pragma omp parallel for schedule(static) if (m > (1 << 20))
     for (int64_t i = 0; i < m; ++i) {
         uint32_t x = 0, y = 0;
         for (int l = 0; l < scale; ++l) {
@@ -187,6 +188,28 @@ int64_t tco_host_count_triangles(int32_t nv, const int64_t *off, const int32_t *
                                  int32_t *triNE, int nthreads)
 {
     return host_count_range(off, ind, triNE, 0, nv, nthreads);
+}
+
+/* The per-entry body of hostCountTriangles (tests/cct.cpp:79-86) on a sample
+ * of entry indices: the CPU-baseline leg times this on uniformly drawn
+ * entries.  Returns the int64 sum of the sampled intersections. */
+int64_t tco_host_count_entries(int32_t nv, const int64_t *off, const int32_t *ind,
+                               const int64_t *entries, int64_t n, int nthreads)
+{
+    int64_t sum = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : sum) num_threads(nthreads) if (nthreads > 1)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t e = entries[i];
+        int32_t lo = 0, hi = nv - 1; /* row holding entry e */
+        while (lo < hi) {
+            int32_t mid = lo + ((hi - lo + 1) >> 1);
+            if (off[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        int32_t src = lo, dest = ind[e];
+        sum += tco_host_single_intersection(src, off[src + 1] - off[src], ind + off[src], dest,
+                                            off[dest + 1] - off[dest], ind + off[dest]);
+    }
+    return sum;
 }
 
 int64_t tco_host_count_rows(int32_t nv, const int64_t *off, const int32_t *ind,

@@ -56,6 +56,9 @@ int64_t tco_host_count_triangles(int32_t nv, const int64_t *off, const int32_t *
 int64_t tco_host_count_rows(int32_t nv, const int64_t *off, const int32_t *ind,
                             int32_t v0, int32_t v1, int nthreads);
 
+/* The per-entry body of hostCountTriangles on sampled entry indices. */
+int64_t tco_host_count_entries(int32_t nv, const int64_t *off, const int32_t *ind,
+                               const int64_t *entries, int64_t n, int nthreads);
 /* devicecuStingerAllTriangles semantics (src/static_triangle_counting/cct.cu:216-270)
  * at its exact (tpi=1) behaviour: out[src] = sum over entries dest of src with
  * src!=dest, deg(dest)>=2, deg(src)>=2 of |N(src) cap N(dest)|, mod 2^32. */

@@ -1,0 +1,236 @@
+"""Pins the CPU oracle (no GPU): closed forms, independent libraries' counts,
+brute force on tiny graphs, and the per-pass accounting of the streaming
+kernels by the number k of new edges in each new triangle."""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from math import comb
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import graphs as GR
+
+
+# ------------------------------------------------------------- generators
+def test_rng_is_counter_based_and_stable():
+    a = [O.rng(4, i) for i in range(8)]
+    assert a == [O.rng(4, i) for i in range(8)]
+    assert len(set(a)) == 8
+    assert O.rng(4, 0) != O.rng(5, 0)
+    # pinned value of the generator spec (the device generator is checked
+    # against the same graphs in the GPU tests)
+    assert O.rng(0, 0) == 0x1094DB8BB4B47185
+
+
+def test_rmat_quadrant_frequencies():
+    u, v = O.rmat_pairs(1, 400000, 9)
+    q = np.bincount(2 * u + v, minlength=4) / len(u)
+    assert np.allclose(q, [0.57, 0.19, 0.19, 0.05], atol=0.005)
+
+
+def test_er_pairs_uniform():
+    u, v = O.er_pairs(10, 200000, 3)
+    h = np.bincount(u, minlength=10) / len(u)
+    assert np.allclose(h, 0.1, atol=0.005)
+    assert u.min() >= 0 and u.max() < 10 and v.max() < 10
+
+
+def test_canonical_csr_properties():
+    u, v = O.rmat_pairs(10, 20000, 2)
+    off, adj = O.canon_csr(1 << 10, u, v)
+    n = len(off) - 1
+    pairs = set()
+    for a in range(n):
+        row = adj[off[a]:off[a + 1]]
+        assert np.all(np.diff(row) > 0)
+        assert a not in row
+        pairs.update((a, int(b)) for b in row)
+    assert all((b, a) in pairs for a, b in pairs)
+    want = {(int(a), int(b)) for a, b in zip(u, v) if a != b} | \
+        {(int(b), int(a)) for a, b in zip(u, v) if a != b}
+    assert pairs == want
+
+
+def test_fixture_hashes_reproduce():
+    meta = json.load(open(os.path.join(GR.GOLDEN, "fixtures.json")))
+    import hashlib
+    off, adj = O.er_graph(1 << 16, 16, 1)
+    assert hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest() == meta["er16"]["csr_sha256"]
+    off, adj = O.rmat_graph(16, 16, 2)
+    assert hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest() == meta["rmat16"]["csr_sha256"]
+
+
+# --------------------------------------------------------- count formulas
+@pytest.mark.parametrize("name", sorted(GR.closed_form_cases()))
+def test_closed_forms(name):
+    off, adj, t = GR.closed_form_cases()[name]
+    want = (2 * t).astype(np.uint32)
+    assert np.array_equal(O.custinger_count(off, adj), want)
+    T, tv = O.forward_count(off, adj, per_vertex=True)
+    assert T == int(t.sum()) // 3
+    assert np.array_equal(tv, t.astype(np.uint64))
+    total, tri = O.host_count_triangles(off, adj, per_edge=True)
+    assert total == 2 * int(t.sum())  # 6T
+    assert int(O.csr_count(off, adj).astype(np.int64).sum()) == int(t.sum())  # 3T
+
+
+def test_complete_graph_closed_form_values():
+    for n in (3, 5, 9, 33):
+        off, adj, _ = GR.complete(n)
+        assert np.all(O.custinger_count(off, adj) == (n - 1) * (n - 2))
+        assert O.forward_count(off, adj)[0] == comb(n, 3)
+
+
+@pytest.mark.parametrize("name", sorted(GR.fixture_cases()))
+def test_against_networkx_fixtures(name):
+    off, adj, t = GR.fixture_cases()[name]
+    assert np.array_equal(O.custinger_count(off, adj), (2 * t).astype(np.uint32))
+    assert np.array_equal(O.forward_count(off, adj, per_vertex=True)[1], t.astype(np.uint64))
+
+
+def test_er16_golden_vector():
+    z = np.load(os.path.join(GR.GOLDEN, "fixtures.npz"))
+    off, adj = O.er_graph(1 << 16, 16, 1)
+    assert np.array_equal(O.custinger_count(off, adj), z["er16_pv"])
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("loops", [False, True])
+def test_reference_formula_brute_force(seed, loops):
+    """Asymmetric rows and self-loops follow the kernel's filters exactly."""
+    off, adj = GR.random_lists(40, 0.2, seed, loops=loops)
+    assert np.array_equal(O.custinger_count(off, adj), GR.brute_custinger(off, adj))
+
+
+def test_planted_mistake_is_caught():
+    """A plausible slip (dropping the deg>=2 filter of cct.cu:247) changes the
+    output: vertex 0 has degree 1 and its neighbour 1 holds a self-loop, so
+    |N(0) cap N(1)| = 1 but the filter makes out[0] = 0."""
+    lists = [[1], [0, 1]]
+    off, adj = O.csr_from_lists(lists)
+    ref = GR.brute_custinger(off, adj)
+    n = len(off) - 1
+    N = [set(adj[off[v]:off[v + 1]].tolist()) for v in range(n)]
+    no_filter = np.array([sum(len(N[s] & N[int(d)]) for d in adj[off[s]:off[s + 1]] if s != d)
+                          for s in range(n)], np.uint32)
+    assert np.array_equal(O.custinger_count(off, adj), ref)
+    assert not np.array_equal(ref, no_filter)
+
+
+def test_single_intersection_against_sets():
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        a = np.unique(rng.integers(0, 200, rng.integers(0, 40))).astype(np.int32)
+        b = np.unique(rng.integers(0, 200, rng.integers(0, 40))).astype(np.int32)
+        got = O.lib().tco_host_single_intersection(0, len(a), O._nz(a), 1, len(b), O._nz(b))
+        assert got == len(set(a.tolist()) & set(b.tolist()))
+
+
+def test_sum_triangle_array_sign_extends():
+    h = np.array([2**31 + 5, 7], np.uint32)  # the first wraps negative as int32
+    assert O.sum_triangle_array(h) == -(2**31) + 5 + 7
+
+
+def test_entry_sample_equals_full_count():
+    off, adj = O.rmat_graph(10, 8, 3)
+    e = np.arange(len(adj), dtype=np.int64)
+    assert O.host_count_entries(off, adj, e, 2) == O.host_count_triangles(off, adj)[0]
+
+
+# ---------------------------------------------------------------- streaming
+def _classify(n, old_edges, new_edges):
+    """Brute force: new triangles grouped by k = number of new edges."""
+    E = set(old_edges) | set(new_edges)
+    new = set(new_edges)
+    adj = [set() for _ in range(n)]
+    for a, b in E:
+        adj[a].add(b)
+        adj[b].add(a)
+    by_k = {1: [], 2: [], 3: []}
+    for a, b, c in itertools.combinations(range(n), 3):
+        if b in adj[a] and c in adj[a] and c in adj[b]:
+            k = sum(tuple(sorted(p)) in new for p in ((a, b), (a, c), (b, c)))
+            if k:
+                by_k[k].append((a, b, c))
+    return by_k
+
+
+def _stream_case(n, old_edges, new_edges):
+    off, adj = O.csr_from_edges(n, old_edges)
+    src = [a for a, b in new_edges] + [b for a, b in new_edges]
+    dst = [b for a, b in new_edges] + [a for a, b in new_edges]
+    s_src, s_dst, dup, boff = O.batch_prepare(n, np.array(src), np.array(dst), off, adj)
+    goff, gadj = O.csr_from_edges(n, list(old_edges) + list(new_edges))
+    return goff, gadj, s_src, s_dst, dup, boff
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_stream_pass_closed_forms_by_k(seed):
+    rng = np.random.default_rng(seed)
+    n = 14
+    allp = [(a, b) for a in range(n) for b in range(a + 1, n)]
+    idx = rng.permutation(len(allp))
+    old = [allp[i] for i in idx[:30]]
+    new = [allp[i] for i in idx[30:48]]
+    by_k = _classify(n, old, new)
+    goff, gadj, s_src, s_dst, dup, boff = _stream_case(n, old, new)
+    sums = {}
+    per = {}
+    for p in (1, 2, 4):
+        out = O.stream_delta(goff, gadj, s_src, s_dst, dup, boff, pass_mask=p)
+        per[p] = out.astype(np.int64)
+        per[p][per[p] >= 2**31] -= 2**32
+        sums[p] = int(per[p].sum())
+    n1, n2, n3 = (len(by_k[k]) for k in (1, 2, 3))
+    assert sums[1] == 6 * (n1 + 2 * n2 + 3 * n3)           # pass 1: +6k
+    assert sums[2] == 6 * n3                                 # pass 2: +6[k=3]
+    assert sums[4] == -6 * (n2 + 3 * n3)                     # pass 3: -6 C(k,2)
+    total = per[1] + per[2] + per[4]
+    want = np.zeros(n, np.int64)
+    for tri in by_k[1] + by_k[2] + by_k[3]:
+        for v in tri:
+            want[v] += 2
+    assert np.array_equal(total, want)
+
+
+def test_stream_delta_equals_recount_with_duplicates():
+    off, adj = O.rmat_graph(9, 8, 11)
+    n = len(off) - 1
+    rng = np.random.default_rng(4)
+    u = rng.integers(0, n, 600).astype(np.int32)
+    v = rng.integers(0, n, 600).astype(np.int32)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    # plant in-batch repeats and already-present edges like the driver does
+    u = np.concatenate([u, u[:40], np.repeat(np.arange(5, dtype=np.int32), 2)])
+    v = np.concatenate([v, v[:40], adj[off[:5]].astype(np.int32).repeat(2)])
+    src = np.concatenate([u, v])
+    dst = np.concatenate([v, u])
+    s_src, s_dst, dup, boff = O.batch_prepare(n, src, dst, off, adj)
+    goff, gadj = O.apply_batch_csr(n, off, adj, ins_u=u, ins_v=v)
+    delta = O.stream_delta(goff, gadj, s_src, s_dst, dup, boff)
+    before = O.custinger_count(off, adj)
+    after = O.custinger_count(goff, gadj)
+    assert np.array_equal(before + delta, after)
+
+
+# ------------------------------------------------------------- load balance
+@pytest.mark.parametrize("n,avg", [(5000, 2), (5000, 5), (5000, 40), (150, 10)])
+def test_lb_partition_properties(n, avg):
+    rng = np.random.default_rng(n + avg)
+    work = rng.poisson(avg, n).astype(np.int64)
+    prefix, parts = O.lb_partition(work)
+    assert np.array_equal(prefix[1:], np.cumsum(work))
+    P = len(parts)
+    a = work.sum() / n
+    expect = 200 if a < 4 else (500 if a < 7 else 1000)
+    assert P == min(expect, n)
+    assert parts[0] == 0 and np.all(np.diff(parts) >= 0) and parts[-1] <= n - 1
+    for b, p in enumerate(parts):
+        needle = int(prefix[n] * (b / P))
+        # lower bound over prefix[0 .. n-1)
+        assert p == np.searchsorted(prefix[:n - 1], needle, side="left")
