@@ -371,8 +371,9 @@ void graph_destroy(Graph *g)
     DevBuf *bufs[] = {&g->st.boff, &g->st.len, &g->st.cap, &g->st.pool, &g->vinfo, &g->padj,
                       &g->coff, &g->crow, &g->work, &g->jobs, &g->tmp, &g->tmp2, &g->tmp3, &g->tmp4,
                       &g->accum, &g->bsrc, &g->bdst, &g->bflag, &g->bkeys, &g->bkeys2,
-                      &g->bvals2, &g->boffb, &g->bdel};
+                      &g->bvals2, &g->boffb, &g->bdel, &g->flagpool, &g->vmark};
     for (DevBuf *b : bufs) b->release();
+    for (DevBuf &b : g->sb) b.release();
     if (g->events_ready)
         for (auto &e : g->ev) hipEventDestroy(e);
     delete g;

@@ -113,6 +113,11 @@ struct Graph {
     // cached derived structures (rebuilt after updates)
     DevBuf vinfo, padj, coff, crow, work, jobs, tmp, tmp2, tmp3, tmp4, accum;
     DevBuf bsrc, bdst, bflag, bkeys, bkeys2, bvals2, boffb, bdel;
+    // streaming: per-entry "in the current batch" flags parallel to the pool,
+    // per-vertex marks, and batch scratch
+    DevBuf flagpool, vmark;
+    DevBuf sb[24];
+    bool degree_stale = false;
     hipEvent_t ev[16] = {};
     bool events_ready = false;
     int ensure_events();

@@ -40,20 +40,73 @@ __device__ inline int64_t readlane_i64(int64_t x, int lane)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Position of w in the LDS owner tile, or -1.  H holds (position << 16) |
-// (low 16 bits of the id) per slot, linear probing, all-ones when empty.
-__device__ inline int32_t hash_find(const uint32_t *H, const int32_t *S, int32_t w, int hb,
-                                    int32_t HS)
-{
-    const uint32_t fp = (uint32_t)w & 0xFFFFu;
-    uint32_t slot = lds_hash(w, hb);
-    while (true) {
-        const uint32_t h = H[slot];
-        if (h == 0xFFFFFFFFu) return -1;
-        if ((h & 0xFFFFu) == fp && S[h >> 16] == w) return (int32_t)(h >> 16);
-        slot = (slot + 1) & (HS - 1);
+// Bucketed two-choice hash table in LDS.  Bucket b holds up to four ids in
+// slots 4b..4b+3, filled as a prefix (inserts claim the first empty slot by
+// CAS), -1 when empty.  An id goes to the less full of its two buckets, and
+// only when both are full to the next non-full bucket after the first.  A
+// lookup reads both buckets (two 16-byte LDS reads) and loops over further
+// buckets only when both are full, which the load factor <= 1/2 makes rare.
+struct BucketTable {
+    int4 *HB;
+    int nbb;  // log2(number of buckets)
+
+    __device__ inline uint32_t b1(int32_t w) const
+    {
+        return nbb ? (((uint32_t)w * 0x9E3779B1u) >> (32 - nbb)) : 0u;
     }
-}
+    __device__ inline uint32_t b2(int32_t w) const
+    {
+        return nbb ? ((((uint32_t)w ^ 0x2C1B3C6Du) * 0x297A2D39u) >> (32 - nbb)) : 0u;
+    }
+    __device__ inline int fill(const int4 x) const
+    {
+        return (x.x >= 0) + (x.y >= 0) + (x.z >= 0) + (x.w >= 0);
+    }
+    __device__ inline bool claim(uint32_t b, int32_t w) const
+    {
+        int *s = reinterpret_cast<int *>(&HB[b]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (atomicCAS(&s[k], -1, w) == -1) return true;
+        return false;
+    }
+    __device__ inline void insert(int32_t w) const
+    {
+        const uint32_t m = (1u << nbb) - 1;
+        const uint32_t x = b1(w), y = b2(w);
+        const bool first = fill(HB[x]) <= fill(HB[y]);
+        if (claim(first ? x : y, w)) return;
+        if (claim(first ? y : x, w)) return;
+        for (uint32_t b = (x + 1) & m;; b = (b + 1) & m)
+            if (claim(b, w)) return;
+    }
+    __device__ inline int32_t match(const int4 v, uint32_t b, int32_t w) const
+    {
+        const int32_t base = (int32_t)(b << 2);
+        return v.x == w ? base : v.y == w ? base + 1 : v.z == w ? base + 2 : v.w == w ? base + 3 : -1;
+    }
+    // Slot of w, or -1 (w < 0 is never present).
+    __device__ inline int32_t find(int32_t w) const
+    {
+        const uint32_t x = b1(w), y = b2(w);
+        const int4 vx = HB[x], vy = HB[y];
+        int32_t s = match(vx, x, w);
+        if (s < 0) s = match(vy, y, w);
+        if (s < 0 && w >= 0 && vx.w >= 0 && vy.w >= 0) {
+            const uint32_t m = (1u << nbb) - 1;
+            for (uint32_t b = (x + 1) & m;; b = (b + 1) & m) {
+                const int4 v = HB[b];
+                s = match(v, b, w);
+                if (s >= 0 || v.w < 0) break;
+            }
+        }
+        return w >= 0 ? s : -1;  // empty slots hold -1
+    }
+    __device__ inline int32_t id_at(int32_t slot) const
+    {
+        return reinterpret_cast<const int32_t *>(HB)[slot];
+    }
+};
 
 // ------------------------------------------------------------ helpers --
 
@@ -442,14 +495,13 @@ template <int LMAX, int NW, bool ORIENTED>
 __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jobs, int64_t njobs,
                                                       CountArgs A)
 {
-    constexpr int HMAX = 2 * LMAX;
-    constexpr int HBMAX = (HMAX <= 1) ? 1 : 32 - __builtin_clz(HMAX - 1);
-    static_assert(LMAX < 65535, "slot positions are 16-bit");
-    // S: owner-list tile (sorted); H: hash slots holding (position << 16 | low 16 bits of id),
-    // 0xFFFFFFFF when empty; C: per-position hit counters (third-vertex counts).
-    __shared__ int32_t S[LMAX];
-    __shared__ uint32_t H[HMAX];
-    __shared__ uint32_t C[ORIENTED ? LMAX : 1];
+    // The owner tile lives in a bucketed two-choice hash table: 4 ids per
+    // 16-byte bucket, load <= 1/2, so a lookup is two ds_read_b128 and eight
+    // compares with no probe loop unless both buckets are full.  C counts hits
+    // per slot (third-vertex counts of the oriented method).
+    constexpr int NBMAX = (LMAX + 1) / 2 > 0 ? (LMAX + 1) / 2 : 1;
+    __shared__ int4 HB[NBMAX];
+    __shared__ uint32_t C[ORIENTED ? 4 * NBMAX : 1];
     __shared__ int32_t pre[NW][kWave + 1];
     __shared__ int64_t tpb[NW][kWave];
     __shared__ int32_t tpl[NW][kWave];
@@ -469,23 +521,17 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         const int32_t s0 = J.tile * LMAX;
         const int32_t L = min(LMAX, Lfull - s0);
         const bool tiled = Lfull > LMAX;
-        int hb = 1;
-        while ((1 << hb) < 2 * L) ++hb;
-        if (hb > HBMAX) hb = HBMAX;
-        const int32_t HS = 1 << hb;
-        for (int i = threadIdx.x; i < L; i += nthr) S[i] = A.adj[obase + s0 + i];
-        for (int i = threadIdx.x; i < HS; i += nthr) H[i] = 0xFFFFFFFFu;
+        const int32_t *Sg = A.adj + obase + s0;  // owner tile (sorted) in global memory
+        int nbb = 0;
+        while ((1 << nbb) * 2 < L) ++nbb;         // buckets: 2^nbb >= L/2
+        const BucketTable T{HB, nbb};
+        for (int i = threadIdx.x; i < (1 << nbb); i += nthr) HB[i] = make_int4(-1, -1, -1, -1);
         if (ORIENTED)
-            for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
+            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) C[i] = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < L; i += nthr) {
-            const int32_t w = S[i];
-            const uint32_t tag = ((uint32_t)i << 16) | ((uint32_t)w & 0xFFFFu);
-            uint32_t slot = lds_hash(w, hb);
-            while (atomicCAS(&H[slot], 0xFFFFFFFFu, tag) != 0xFFFFFFFFu) slot = (slot + 1) & (HS - 1);
-        }
+        for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
         __syncthreads();
-        const int32_t lo_v = S[0], hi_v = S[L - 1];
+        const int32_t lo_v = Sg[0], hi_v = Sg[L - 1];
         unsigned long long hits_job = 0;
 
         // tasks are dealt to the waves round-robin so short jobs still keep every wave busy
@@ -527,13 +573,13 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
 
             // Long hash tasks run one at a time across the whole wave: the list
             // base is wave-uniform (scalar registers), each lane streams every
-            // 64th entry through the LDS hash, and hits stay in registers.
+            // 64th entry through the table, and hits stay in registers.
             uint64_t big = __ballot(strat == S_HASH && pl >= kBigTask);
             while (big) {
-                const int jb = __ffsll((unsigned long long)big) - 1;
+                const int jb2 = __ffsll((unsigned long long)big) - 1;
                 big &= big - 1;
-                const int32_t *lst = A.adj + readlane_i64(pb, jb);
-                const int32_t n = __builtin_amdgcn_readlane(pl, jb);
+                const int32_t *lst = A.adj + readlane_i64(pb, jb2);
+                const int32_t n = __builtin_amdgcn_readlane(pl, jb2);
                 uint32_t h = 0;
                 for (int32_t k0 = lane; k0 < n; k0 += kUnroll * kWave) {
                     int32_t w[kUnroll];
@@ -544,23 +590,21 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                     }
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
-                        if (w[u] < 0) continue;
-                        const int32_t pos = hash_find(H, S, w[u], hb, HS);
-                        if (pos >= 0) {
+                        const int32_t slot = T.find(w[u]);
+                        if (slot >= 0) {
                             ++h;
-                            if (ORIENTED) atomicAdd(&C[pos], 1u);
+                            if (ORIENTED) atomicAdd(&C[slot], 1u);
                         }
                     }
                 }
                 h = wave_sum<uint32_t>(h);
-                if (lane == jb) tcnt[wid][jb] = h;
+                if (lane == jb2) tcnt[wid][jb2] = h;
             }
 
-            // Two flattened phases over the 64 tasks of this wave: items of the
-            // hash phase are task-list entries, items of the search phase are
-            // owner-list entries.  Each lane keeps a running task index and a
-            // private hit count, flushed when its task changes, and works on
-            // kUnroll items at once so that their loads overlap.
+            // Flattened phases over the remaining tasks of this wave: items of
+            // the hash phase are task-list entries, items of the search phase
+            // are owner-tile entries.  Each lane keeps a running task index and
+            // a private hit count, flushed when its task changes.
 #pragma unroll 1
             for (int phase = 0; phase < 2; ++phase) {
                 const int want = phase == 0 ? S_HASH : S_BSEARCH;
@@ -584,7 +628,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                         ok[u] = it < total;
                         jj[u] = j;
                         kk[u] = 0;
-                        ww[u] = 0;
+                        ww[u] = -1;
                         if (ok[u]) {
                             if (pre[wid][j + 1] <= it) {
                                 int32_t lo = j + 1, hi = kWave;  // largest j with pre[j] <= it
@@ -596,33 +640,32 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                             }
                             jj[u] = j;
                             kk[u] = it - pre[wid][j];
-                            if (phase == 0) ww[u] = A.adj[tpb[wid][j] + kk[u]];
+                            ww[u] = (phase == 0) ? A.adj[tpb[wid][j] + kk[u]] : Sg[kk[u]];
                         }
                     }
                     if (phase == 0) {
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
+                            const int32_t slot = T.find(ww[u]);
                             if (!ok[u]) continue;
                             if (jj[u] != myj) {
                                 if (myhits) atomicAdd(&tcnt[wid][myj], myhits);
                                 myj = jj[u];
                                 myhits = 0;
                             }
-                            const int32_t pos = hash_find(H, S, ww[u], hb, HS);
-                            if (pos >= 0) {
+                            if (slot >= 0) {
                                 ++myhits;
-                                if (ORIENTED) atomicAdd(&C[pos], 1u);
+                                if (ORIENTED) atomicAdd(&C[slot], 1u);
                             }
                         }
                     } else {
                         // interleaved lower_bound searches of owner entries in task lists
                         const int32_t *lst[kUnroll];
-                        int32_t key[kUnroll], b0[kUnroll], n0[kUnroll], nn[kUnroll];
+                        int32_t b0[kUnroll], n0[kUnroll], nn[kUnroll];
                         int32_t steps = 0;
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
                             lst[u] = A.adj + tpb[wid][jj[u]];
-                            key[u] = S[kk[u]];
                             n0[u] = ok[u] ? tpl[wid][jj[u]] : 0;
                             nn[u] = n0[u];
                             b0[u] = 0;
@@ -633,7 +676,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                             for (int u = 0; u < kUnroll; ++u) {
                                 if (nn[u] > 0) {
                                     const int32_t half = nn[u] >> 1;
-                                    if (lst[u][b0[u] + half] < key[u]) {
+                                    if (lst[u][b0[u] + half] < ww[u]) {
                                         b0[u] += half + 1;
                                         nn[u] -= half + 1;
                                     } else {
@@ -650,9 +693,9 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                                 myj = jj[u];
                                 myhits = 0;
                             }
-                            if (b0[u] < n0[u] && lst[u][b0[u]] == key[u]) {
+                            if (b0[u] < n0[u] && lst[u][b0[u]] == ww[u]) {
                                 ++myhits;
-                                if (ORIENTED) atomicAdd(&C[kk[u]], 1u);
+                                if (ORIENTED) atomicAdd(&C[T.find(ww[u])], 1u);
                             }
                         }
                     }
@@ -667,15 +710,15 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                     int32_t first = lst[0], a = 0, b = L;
                     while (a < b) {
                         int32_t mid = (a + b) >> 1;
-                        if (S[mid] < first) a = mid + 1; else b = mid;
+                        if (Sg[mid] < first) a = mid + 1; else b = mid;
                     }
                     k = a;
                 }
                 while (i < pl && k < L) {
-                    int32_t x = lst[i], y = S[k];
+                    int32_t x = lst[i], y = Sg[k];
                     if (x == y) {
                         ++mcount;
-                        if (ORIENTED) atomicAdd(&C[k], 1u);
+                        if (ORIENTED) atomicAdd(&C[T.find(x)], 1u);
                     }
                     i += (x <= y);
                     k += (x >= y);
@@ -696,9 +739,9 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         unsigned long long hjob = 0;
         for (int w = 0; w < NW; ++w) hjob += red[w];
         if (ORIENTED) {
-            for (int i = threadIdx.x; i < L; i += nthr) {
-                uint32_t ci = C[i];
-                if (ci) atomicAdd(&A.out[S[i]], 2u * ci);
+            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
+                const uint32_t ci = C[i];
+                if (ci) atomicAdd(&A.out[T.id_at(i)], 2u * ci);
             }
         }
         if (threadIdx.x == 0 && hjob)

@@ -1,0 +1,845 @@
+// tc_stream.hip -- batch updates of the dynamic store and the streaming
+// triangle-count delta.
+//
+// Replaces the reference's batch pipeline (BatchUpdate::sortDeviceBUD +
+// cuStinger::edgeInsertionsSorted / edgeDeletions, tests/cct_upd.cpp:391-398,
+// tests/katz-tester.cu:244) and callDeviceNewTriangles
+// (src/streaming_triangle_counting/cct_upd.cu:412-440).
+//
+// Delta method.  Let B be the batch's effective new edges, all present in the
+// graph G* (the post-insert graph for insertions, the pre-delete graph for
+// deletions).  The reference runs three passes over directed batch edges:
+// +1 to s, d and w for w in N*(s) & N*(d); +1 to s for w in NB(s) & NB(d);
+// -1 to s, d, w for w in NB(s) & N*(d).  Folding the three passes per
+// undirected edge (s,d) and common neighbour w, with bs = [(s,w) in B] and
+// bd = [(d,w) in B], gives s and d each 1 + [bs = bd = 0] and w 2 - bs - bd;
+// over a triangle with k = 1, 2 or 3 new edges this sums to exactly +2 per
+// corner, the reference's result.  One intersection per undirected new edge;
+// membership in B is a per-entry flag kept beside the adjacency pool.
+#include "tc_internal.hpp"
+
+#include <algorithm>
+
+namespace tc {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline int32_t wave_scan(int32_t x, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        int32_t y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__device__ inline int64_t find_pos(const int32_t *__restrict__ pool, int64_t base, int32_t n,
+                                   int32_t key)
+{
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (pool[base + mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return (lo < n && pool[base + lo] == key) ? base + lo : -1;
+}
+
+__global__ void k_batch_check(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
+                              int64_t B, int32_t nv, unsigned long long *__restrict__ bad)
+{
+    unsigned long long oor = 0, loops = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t a = u[i], b = v[i];
+        if (a < 0 || a >= nv || b < 0 || b >= nv) ++oor;
+        else if (a == b) ++loops;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        oor += __shfl_xor(oor, d);
+        loops += __shfl_xor(loops, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (oor) atomicAdd(&bad[0], oor);
+        if (loops) atomicAdd(&bad[1], loops);
+    }
+}
+
+// key = (min << (bv+1)) | (max << 1) | is_delete
+__global__ void k_batch_keys(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
+                             const uint8_t *__restrict__ del, int64_t B, int bv,
+                             uint64_t *__restrict__ keys)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t a = (uint32_t)min(u[i], v[i]), b = (uint32_t)max(u[i], v[i]);
+        keys[i] = (a << (bv + 1)) | (b << 1) | (uint64_t)((del && del[i]) ? 1 : 0);
+    }
+}
+
+__global__ void k_unique_split(const uint64_t *__restrict__ keys, int64_t n,
+                               int32_t *__restrict__ fi, int32_t *__restrict__ fd)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = keys[i];
+        bool first = i == 0 || keys[i - 1] != k;
+        fi[i] = first && !(k & 1);
+        fd[i] = first && (k & 1);
+    }
+}
+
+__global__ void k_compact_pairs(const uint64_t *__restrict__ keys, int64_t n, int bv,
+                                const int32_t *__restrict__ fi, const int64_t *__restrict__ pi,
+                                const int32_t *__restrict__ fd, const int64_t *__restrict__ pd,
+                                int32_t *__restrict__ ia, int32_t *__restrict__ ib,
+                                int32_t *__restrict__ da, int32_t *__restrict__ db)
+{
+    const uint64_t m = (1ull << bv) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = keys[i];
+        int32_t a = (int32_t)(k >> (bv + 1)), b = (int32_t)((k >> 1) & m);
+        if (fi[i]) {
+            ia[pi[i]] = a;
+            ib[pi[i]] = b;
+        }
+        if (fd[i]) {
+            da[pd[i]] = a;
+            db[pd[i]] = b;
+        }
+    }
+}
+
+// Locates (a,b) and (b,a) in the store; sel = 1 where the pair is present.
+__global__ void k_find_pairs(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+                             int64_t n, const int32_t *__restrict__ pool,
+                             const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
+                             int64_t *__restrict__ pos_ab, int64_t *__restrict__ pos_ba,
+                             int32_t *__restrict__ present)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t a = pa[i], b = pb[i];
+        int64_t x = find_pos(pool, boff[a], len[a], b);
+        int64_t y = find_pos(pool, boff[b], len[b], a);
+        pos_ab[i] = x;
+        pos_ba[i] = y;
+        present[i] = (x >= 0 && y >= 0) ? 1 : 0;
+    }
+}
+
+__global__ void k_set_flags(const int64_t *__restrict__ pos_ab, const int64_t *__restrict__ pos_ba,
+                            const int32_t *__restrict__ sel, int64_t n, uint8_t val,
+                            uint8_t *__restrict__ flag)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (sel && !sel[i]) continue;
+        if (pos_ab[i] >= 0) flag[pos_ab[i]] = val;
+        if (pos_ba[i] >= 0) flag[pos_ba[i]] = val;
+    }
+}
+
+__global__ void k_not(const int32_t *__restrict__ in, int64_t n, int32_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i] ? 0 : 1;
+}
+
+// Marks each endpoint of selected pairs once; unmarked vertices are queued.
+__global__ void k_queue_endpoints(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+                                  const int32_t *__restrict__ sel, int64_t n,
+                                  int32_t *__restrict__ mark, int32_t *__restrict__ queue,
+                                  unsigned long long *__restrict__ qlen)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!sel[i]) continue;
+        int32_t e[2] = {pa[i], pb[i]};
+        for (int q = 0; q < 2; ++q)
+            if (atomicExch(&mark[e[q]], 1) == 0) queue[atomicAdd(qlen, 1ull)] = e[q];
+    }
+}
+
+__global__ void k_unmark(const int32_t *__restrict__ queue, int64_t n, int32_t *__restrict__ mark)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        mark[queue[i]] = 0;
+}
+
+// Removes flagged entries from each queued vertex's block (one wave per
+// vertex, order preserving, in place).
+__global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n,
+                               const int64_t *__restrict__ boff, int32_t *__restrict__ len,
+                               int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n;
+         q += wstride) {
+        const int32_t v = queue[q];
+        const int64_t base = boff[v];
+        const int32_t l = len[v];
+        int32_t w = 0;
+        for (int32_t r0 = 0; r0 < l; r0 += kWave) {
+            const int32_t r = r0 + lane;
+            int32_t x = 0;
+            bool keep = false;
+            if (r < l) {
+                x = pool[base + r];
+                keep = flag[base + r] == 0;
+            }
+            const uint64_t m = __ballot(keep);
+            __builtin_amdgcn_wave_barrier();
+            if (keep) {
+                const int32_t dst = w + __popcll(m & lt);
+                pool[base + dst] = x;
+                flag[base + dst] = 0;
+            }
+            w += __popcll(m);
+        }
+        for (int32_t r = w + lane; r < l; r += kWave) flag[base + r] = 0;
+        if (lane == 0) len[v] = w;
+    }
+}
+
+// Directed keys (src << bv | dst) of the selected pairs, both directions.
+__global__ void k_directed_keys(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+                                const int64_t *__restrict__ sidx, int64_t n,
+                                const int32_t *__restrict__ sel, int bv, uint64_t *__restrict__ keys)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!sel[i]) continue;
+        uint64_t a = (uint32_t)pa[i], b = (uint32_t)pb[i];
+        int64_t p = 2 * sidx[i];
+        keys[p] = (a << bv) | b;
+        keys[p + 1] = (b << bv) | a;
+    }
+}
+
+__global__ void k_segment_heads(const uint64_t *__restrict__ keys, int64_t n, int bv,
+                                int32_t *__restrict__ head)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0) || ((keys[i] >> bv) != (keys[i - 1] >> bv));
+}
+
+// Per affected vertex: its segment in the sorted directed keys, the new length
+// and the block size it needs when it must move.
+__global__ void k_segments(const uint64_t *__restrict__ keys, const int32_t *__restrict__ head,
+                           const int64_t *__restrict__ hpos, int64_t n, int bv, int64_t nseg,
+                           const int32_t *__restrict__ len, const int32_t *__restrict__ cap,
+                           int32_t *__restrict__ sv, int64_t *__restrict__ sbeg,
+                           int32_t *__restrict__ need)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!head[i]) continue;
+        int64_t s = hpos[i];
+        int32_t v = (int32_t)(keys[i] >> bv);
+        sv[s] = v;
+        sbeg[s] = i;
+        (void)nseg;
+    }
+}
+
+__global__ void k_need(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg,
+                       int64_t n, const int32_t *__restrict__ len, const int32_t *__restrict__ cap,
+                       int32_t *__restrict__ need)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        int64_t cnt = ((s + 1 < nseg) ? sbeg[s + 1] : n) - sbeg[s];
+        int64_t nl = (int64_t)len[sv[s]] + cnt;
+        int32_t c = 0;
+        if (nl > cap[sv[s]]) {
+            int64_t p = 1;
+            while (p < nl) p <<= 1;
+            c = (int32_t)p;
+        }
+        need[s] = c;
+    }
+}
+
+// Merges a vertex's sorted new entries into its block (one wave per vertex).
+// Phase 1 computes each new entry's final slot from the untouched old list;
+// phase 2 moves the old entries (into a fresh block, or in place from the
+// back in 64-wide chunks: a chunk's reads precede its writes and its
+// destinations never fall below the chunk); phase 3 writes the new entries
+// into the remaining slots with flag 1 and leaves their slots in newpos.
+__global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg,
+                             int64_t nseg, int64_t n, const uint64_t *__restrict__ keys, int bv,
+                             const int32_t *__restrict__ need, const int64_t *__restrict__ nscan,
+                             int64_t pool_used, int64_t *__restrict__ boff,
+                             int32_t *__restrict__ len, int32_t *__restrict__ cap,
+                             int32_t *__restrict__ pool, uint8_t *__restrict__ flag,
+                             int64_t *__restrict__ newpos)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t m = (1ull << bv) - 1;
+    for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; s < nseg;
+         s += wstride) {
+        const int32_t v = sv[s];
+        const int64_t b0 = sbeg[s];
+        const int32_t cn = (int32_t)(((s + 1 < nseg) ? sbeg[s + 1] : n) - b0);
+        const int32_t lo = len[v];
+        const int64_t ob = boff[v];
+        const bool move = need[s] > 0;
+        const int64_t db = move ? pool_used + nscan[s] : ob;
+        // phase 1: final slots of the new entries
+        for (int32_t i = lane; i < cn; i += kWave) {
+            const int32_t x = (int32_t)(keys[b0 + i] & m);
+            int32_t a = 0, b = lo;
+            while (a < b) {
+                int32_t mid = (a + b) >> 1;
+                if (pool[ob + mid] < x) a = mid + 1; else b = mid;
+            }
+            newpos[b0 + i] = db + i + a;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // phase 2: old entries shift up by the number of new entries below them
+        if (move) {
+            for (int32_t j = lane; j < lo; j += kWave) {
+                const int32_t y = pool[ob + j];
+                int32_t a = 0, b = cn;
+                while (a < b) {
+                    int32_t mid = (a + b) >> 1;
+                    if ((int32_t)(keys[b0 + mid] & m) < y) a = mid + 1; else b = mid;
+                }
+                pool[db + j + a] = y;
+                flag[db + j + a] = 0;
+            }
+        } else {
+            const int32_t top = ((lo + kWave - 1) / kWave) * kWave;
+            for (int32_t j0 = top - kWave; j0 >= 0; j0 -= kWave) {
+                const int32_t j = j0 + lane;
+                int32_t y = 0, a = 0;
+                if (j < lo) {
+                    y = pool[ob + j];
+                    int32_t lo2 = 0, hi2 = cn;
+                    while (lo2 < hi2) {
+                        int32_t mid = (lo2 + hi2) >> 1;
+                        if ((int32_t)(keys[b0 + mid] & m) < y) lo2 = mid + 1; else hi2 = mid;
+                    }
+                    a = lo2;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (j < lo && a > 0) {
+                    pool[ob + j + a] = y;
+                    flag[ob + j + a] = 0;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // phase 3: new entries into the gaps
+        for (int32_t i = lane; i < cn; i += kWave) {
+            const int64_t d = newpos[b0 + i];
+            pool[d] = (int32_t)(keys[b0 + i] & m);
+            flag[d] = 1;
+        }
+        if (lane == 0) {
+            len[v] = lo + cn;
+            if (move) {
+                boff[v] = db;
+                cap[v] = need[s];
+            }
+        }
+    }
+}
+
+__global__ void k_clear_flags(const int64_t *__restrict__ pos, int64_t n, uint8_t *__restrict__ flag)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        flag[pos[i]] = 0;
+}
+
+// Delta over undirected batch pairs present in G* (sel[i] != 0): the shorter
+// list of s and d is searched entry by entry in the longer one; each common
+// neighbour w adds sign*(2 - bs - bd) to w and sign*(1 + [bs=bd=0]) to s and d.
+// Tasks are flattened across the 64 lanes; hits stay in registers per task.
+__global__ __launch_bounds__(kThreads) void k_delta(
+    const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+    const int32_t *__restrict__ sel, int64_t n, const int32_t *__restrict__ pool,
+    const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
+    const uint8_t *__restrict__ flag, int32_t sign, uint32_t *__restrict__ out,
+    unsigned long long *__restrict__ gsum)
+{
+    constexpr int NW = kThreads / kWave;
+    __shared__ int32_t pre[NW][kWave + 1];
+    __shared__ int64_t tsb[NW][kWave], tlb[NW][kWave];
+    __shared__ int32_t tll[NW][kWave];
+    __shared__ uint8_t tss[NW][kWave];
+    __shared__ uint32_t tcnt[NW][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int64_t nwaves = (int64_t)gridDim.x * NW;
+    int64_t local = 0;
+    for (int64_t g0 = ((int64_t)blockIdx.x * NW + wid) * kWave; g0 < n; g0 += nwaves * kWave) {
+        const int64_t t = g0 + lane;
+        bool valid = t < n && (sel == nullptr || sel[t]);
+        int32_t s = 0, d = 0, ls = 0, ld = 0;
+        if (valid) {
+            s = pa[t];
+            d = pb[t];
+            ls = len[s];
+            ld = len[d];
+        }
+        const bool s_short = ls <= ld;
+        tsb[wid][lane] = valid ? (s_short ? boff[s] : boff[d]) : 0;
+        tlb[wid][lane] = valid ? (s_short ? boff[d] : boff[s]) : 0;
+        tll[wid][lane] = s_short ? ld : ls;
+        tss[wid][lane] = s_short ? 1 : 0;
+        tcnt[wid][lane] = 0;
+        const int32_t items = valid ? min(ls, ld) : 0;
+        const int32_t incl = wave_scan(items, lane);
+        pre[wid][lane + 1] = incl;
+        if (lane == 0) pre[wid][0] = 0;
+        const int32_t total = __shfl(incl, kWave - 1, kWave);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int32_t j = 0, myj = -1;
+        uint32_t mine = 0;
+        for (int32_t it = lane; it < total; it += kWave) {
+            if (pre[wid][j + 1] <= it) {
+                int32_t lo = j + 1, hi = kWave;
+                while (hi - lo > 1) {
+                    int32_t mid = (lo + hi) >> 1;
+                    if (pre[wid][mid] <= it) lo = mid; else hi = mid;
+                }
+                j = lo;
+            }
+            if (j != myj) {
+                if (mine) atomicAdd(&tcnt[wid][myj], mine);
+                myj = j;
+                mine = 0;
+            }
+            const int32_t k = it - pre[wid][j];
+            const int64_t sb = tsb[wid][j], lb = tlb[wid][j];
+            const int32_t x = pool[sb + k];
+            const int64_t p = find_pos(pool, lb, tll[wid][j], x);
+            if (p >= 0) {
+                const int fs = flag[sb + k], fl = flag[p];
+                const int bs = tss[wid][j] ? fs : fl;
+                const int bd = tss[wid][j] ? fl : fs;
+                const int cw = 2 - bs - bd;
+                if (cw) atomicAdd(&out[x], (uint32_t)(sign * cw));
+                local += sign * cw;
+                mine += (bs | bd) ? 1u : 2u;
+            }
+        }
+        if (mine) atomicAdd(&tcnt[wid][myj], mine);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t c = tcnt[wid][lane];
+        if (valid && c) {
+            atomicAdd(&out[s], (uint32_t)(sign * (int32_t)c));
+            atomicAdd(&out[d], (uint32_t)(sign * (int32_t)c));
+            local += 2 * (int64_t)sign * c;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int dd = 32; dd > 0; dd >>= 1) local += __shfl_xor(local, dd);
+    if (lane == 0 && local) atomicAdd(gsum, (unsigned long long)local);
+}
+
+// The reference's three passes at one lane per directed batch entry
+// (cct_upd.cu:146-176, 231-410): masked sequential merges, atomics on the
+// third vertex.  Kept for pass-by-pass parity.
+__device__ uint32_t masked_merge_dev(const int32_t *u, int32_t ul, const int32_t *um,
+                                     const int32_t *v, int32_t vl, const int32_t *vm,
+                                     uint32_t *out, int sign)
+{
+    int32_t i = 0, j = 0;
+    uint32_t c = 0;
+    while (i < ul && j < vl) {
+        const int vmask = vm ? vm[j] : 0;
+        const int umask = um ? um[i] : 0;
+        const int32_t cmp = u[i] - v[j];
+        const bool hit = cmp == 0 && !umask && !vmask;
+        c += hit;
+        if (hit && out) atomicAdd(&out[u[i]], (uint32_t)sign);
+        i += ((cmp <= 0 && !vmask) || umask);
+        j += ((cmp >= 0 && !umask) || vmask);
+    }
+    return c;
+}
+
+__global__ void k_ref_passes(int64_t nb, const int32_t *__restrict__ s_src,
+                             const int32_t *__restrict__ s_dst, const int32_t *__restrict__ dup,
+                             const int64_t *__restrict__ bo, const int32_t *__restrict__ pool,
+                             const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
+                             uint32_t *__restrict__ out, int32_t mask)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nb;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t s = s_src[e], d = s_dst[e];
+        if ((mask & 1) && dup[e] != 1) {
+            const int32_t sl = len[s], dl = len[d];
+            if (!(s == d || dl < 2 || sl < 2)) {
+                const bool ss = sl < dl;
+                const int32_t sm = ss ? s : d, lg = ss ? d : s;
+                const uint32_t c = masked_merge_dev(pool + boff[sm], ss ? sl : dl, nullptr,
+                                                    pool + boff[lg], ss ? dl : sl, nullptr, out, 1);
+                atomicAdd(&out[s], c);
+                atomicAdd(&out[d], c);
+            }
+        }
+        if ((mask & 2) && !dup[e]) {
+            const int32_t sl = (int32_t)(bo[s + 1] - bo[s]), dl = (int32_t)(bo[d + 1] - bo[d]);
+            if (!(s == d || dl < 2 || sl < 2)) {
+                const bool ss = sl < dl;
+                const int32_t sm = ss ? s : d, lg = ss ? d : s;
+                const uint32_t c = masked_merge_dev(s_dst + bo[sm], ss ? sl : dl, dup + bo[sm],
+                                                    s_dst + bo[lg], ss ? dl : sl, dup + bo[lg],
+                                                    nullptr, 0);
+                atomicAdd(&out[s], c);
+            }
+        }
+        if ((mask & 4) && !dup[e]) {
+            const int32_t sl = (int32_t)(bo[s + 1] - bo[s]), dl = len[d];
+            if (!(s == d || dl < 2 || sl < 2)) {
+                const int32_t *sp = s_dst + bo[s];
+                const int32_t *smk = dup + bo[s];
+                const int32_t *dp = pool + boff[d];
+                const uint32_t c = (sl < dl) ? masked_merge_dev(sp, sl, smk, dp, dl, nullptr, out, -1)
+                                             : masked_merge_dev(dp, dl, nullptr, sp, sl, smk, out, -1);
+                atomicSub(&out[s], c);
+                atomicSub(&out[d], c);
+            }
+        }
+    }
+}
+
+__global__ void k_zero_u64s(unsigned long long *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_bytes(uint8_t *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+}  // namespace
+
+}  // namespace tc
+
+namespace tc {
+
+namespace {
+
+int ensure_flags(Graph &g)
+{
+    const size_t cap = g.st.pool.bytes / sizeof(int32_t);
+    if (g.flagpool.bytes < cap) {
+        DevBuf nf;
+        TC_TRY(nf.ensure(cap));
+        hipLaunchKernelGGL(k_zero_bytes, dim3(grid_for((int64_t)cap, 256, 65536)), dim3(256), 0,
+                           g.stream, nf.as<uint8_t>(), (int64_t)cap);
+        if (g.flagpool.p && g.st.pool_used > 0)
+            TC_HIP(hipMemcpyAsync(nf.p, g.flagpool.p, (size_t)g.st.pool_used, hipMemcpyDeviceToDevice,
+                                  g.stream));
+        g.flagpool.release();
+        g.flagpool = nf;
+    }
+    if (g.vmark.bytes < (size_t)std::max(g.st.nv, 1) * sizeof(int32_t)) {
+        TC_TRY(g.vmark.ensure((size_t)std::max(g.st.nv, 1) * sizeof(int32_t)));
+        TC_HIP(hipMemsetAsync(g.vmark.p, 0, g.vmark.bytes, g.stream));
+    }
+    return TC_OK;
+}
+
+// Grows the pool (and its flags) so that `extra` more entries fit past pool_used.
+int ensure_pool(Graph &g, int64_t extra)
+{
+    const int64_t cap = (int64_t)(g.st.pool.bytes / sizeof(int32_t));
+    if (g.st.pool_used + extra <= cap) return TC_OK;
+    int64_t ncap = std::max<int64_t>(2 * cap, g.st.pool_used + extra + 1024);
+    DevBuf np, nf;
+    TC_TRY(np.ensure((size_t)ncap * sizeof(int32_t)));
+    int st = nf.ensure((size_t)ncap);
+    if (st != TC_OK) {
+        np.release();
+        return st;
+    }
+    hipLaunchKernelGGL(k_zero_bytes, dim3(grid_for(ncap, 256, 65536)), dim3(256), 0, g.stream,
+                       nf.as<uint8_t>(), ncap);
+    if (g.st.pool_used > 0) {
+        TC_HIP(hipMemcpyAsync(np.p, g.st.pool.p, (size_t)g.st.pool_used * sizeof(int32_t),
+                              hipMemcpyDeviceToDevice, g.stream));
+        TC_HIP(hipMemcpyAsync(nf.p, g.flagpool.p, (size_t)g.st.pool_used, hipMemcpyDeviceToDevice,
+                              g.stream));
+    }
+    TC_HIP(hipStreamSynchronize(g.stream));
+    g.st.pool.release();
+    g.flagpool.release();
+    g.st.pool = np;
+    g.flagpool = nf;
+    return TC_OK;
+}
+
+template <class T> T *buf(Graph &g, int i, int64_t n)
+{
+    if (g.sb[i].ensure((size_t)std::max<int64_t>(n, 1) * sizeof(T)) != TC_OK) return nullptr;
+    return g.sb[i].as<T>();
+}
+
+#define TC_NEED(p)                                                                     \
+    do {                                                                               \
+        if (!(p)) return TC_ERR_OOM;                                                   \
+    } while (0)
+
+int64_t read_i64(const int64_t *d, hipStream_t s)
+{
+    int64_t h = 0;
+    if (hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    return h;
+}
+
+}  // namespace
+
+int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_del, int64_t B,
+                uint32_t *delta, int64_t *dglobal, const tc_schedule &sched, tc_batch_stats *stats,
+                bool count)
+{
+    (void)sched;
+    const int32_t nv = g.st.nv;
+    hipStream_t s = g.stream;
+    tc_batch_stats bs = {};
+    if (dglobal) *dglobal = 0;
+    if (B == 0) {
+        if (stats) *stats = bs;
+        return TC_OK;
+    }
+    if (g.symmetric < 0 || g.self_loops < 0) {
+        int64_t bad = 0;
+        TC_TRY(validate_graph(g, true, &bad));
+    }
+    if (g.symmetric != 1 || g.self_loops != 0) {
+        set_last_error(TC_ERR_INVALID, "batch updates need a symmetric store without self-loops");
+        return TC_ERR_INVALID;
+    }
+    TC_TRY(g.ensure_events());
+    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
+    unsigned long long *acc = g.accum.as<unsigned long long>();
+    hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc, 16);
+    hipLaunchKernelGGL(k_batch_check, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, u, v, B, nv, acc);
+    TC_HIP(hipGetLastError());
+    unsigned long long hb[2];
+    TC_HIP(hipMemcpyAsync(hb, acc, sizeof(hb), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    if (hb[0] || hb[1]) {
+        set_last_error(TC_ERR_INVALID, "batch has " + std::to_string(hb[0]) +
+                                           " out-of-range ids and " + std::to_string(hb[1]) +
+                                           " self-loops; nothing applied");
+        return TC_ERR_INVALID;
+    }
+    TC_TRY(ensure_flags(g));
+    TC_HIP(hipEventRecord(g.ev[8], s));
+
+    int bv = 1;
+    while ((int64_t(1) << bv) < nv) ++bv;
+    const int bits = 2 * bv + 1;
+    // sort (pair, op) keys and split unique inserts / deletes
+    uint64_t *keys = buf<uint64_t>(g, 0, B);
+    TC_NEED(keys);
+    hipLaunchKernelGGL(k_batch_keys, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, u, v, is_del,
+                       B, bv, keys);
+    TC_TRY(radix_sort_u64(keys, nullptr, B, bits, g.sb[1], g.sb[2], g.tmp, s));
+    int32_t *fi = buf<int32_t>(g, 3, B), *fd = buf<int32_t>(g, 4, B);
+    int64_t *pi = buf<int64_t>(g, 5, B + 1), *pd = buf<int64_t>(g, 6, B + 1);
+    TC_NEED(fi && fd && pi && pd);
+    hipLaunchKernelGGL(k_unique_split, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, keys, B, fi, fd);
+    TC_TRY(scan_counts_i32(fi, pi, B, g.tmp, s));
+    TC_TRY(scan_counts_i32(fd, pd, B, g.tmp, s));
+    const int64_t nI = read_i64(pi + B, s), nD = read_i64(pd + B, s);
+    int32_t *ia = buf<int32_t>(g, 7, nI), *ib = buf<int32_t>(g, 8, nI);
+    int32_t *da = buf<int32_t>(g, 9, nD), *db = buf<int32_t>(g, 10, nD);
+    TC_NEED(ia && ib && da && db);
+    hipLaunchKernelGGL(k_compact_pairs, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, keys, B, bv,
+                       fi, pi, fd, pd, ia, ib, da, db);
+    TC_HIP(hipGetLastError());
+
+    uint32_t *out = delta;
+    if (count && !out) {
+        TC_TRY(g.tmp4.ensure((size_t)std::max(nv, 1) * sizeof(uint32_t)));
+        out = g.tmp4.as<uint32_t>();
+    }
+    double ms_delta = 0;
+    float ms = 0;
+
+    // ---------------------------------------------------------- deletions
+    if (nD > 0) {
+        int64_t *pab = buf<int64_t>(g, 11, nD), *pba = buf<int64_t>(g, 12, nD);
+        int32_t *pres = buf<int32_t>(g, 13, nD);
+        int32_t *queue = buf<int32_t>(g, 14, 2 * nD);
+        TC_NEED(pab && pba && pres && queue);
+        hipLaunchKernelGGL(k_find_pairs, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, da, db, nD,
+                           g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                           pab, pba, pres);
+        hipLaunchKernelGGL(k_set_flags, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, pab, pba,
+                           pres, nD, (uint8_t)1, g.flagpool.as<uint8_t>());
+        if (count) {
+            TC_HIP(hipEventRecord(g.ev[9], s));
+            hipLaunchKernelGGL(k_delta, dim3(grid_for(nD, kThreads, 1 << 16)), dim3(kThreads), 0, s,
+                               da, db, pres, nD, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                               g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), -1, out, acc + 2);
+            TC_HIP(hipEventRecord(g.ev[10], s));
+        }
+        hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc + 4, 1);
+        hipLaunchKernelGGL(k_queue_endpoints, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, da, db,
+                           pres, nD, g.vmark.as<int32_t>(), queue, acc + 4);
+        TC_HIP(hipGetLastError());
+        unsigned long long hq = 0;
+        TC_HIP(hipMemcpyAsync(&hq, acc + 4, sizeof(hq), hipMemcpyDeviceToHost, s));
+        TC_HIP(hipStreamSynchronize(s));
+        if (count) {
+            hipEventElapsedTime(&ms, g.ev[9], g.ev[10]);
+            ms_delta += ms;
+        }
+        if (hq > 0) {
+            hipLaunchKernelGGL(k_compact_rows, dim3(grid_for((int64_t)hq, 4, 65536)), dim3(256), 0, s,
+                               queue, (int64_t)hq, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                               g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
+            hipLaunchKernelGGL(k_unmark, dim3(grid_for((int64_t)hq, 256, 65536)), dim3(256), 0, s,
+                               queue, (int64_t)hq, g.vmark.as<int32_t>());
+            TC_HIP(hipGetLastError());
+        }
+        // deleted = present pairs
+        int64_t *cnt = buf<int64_t>(g, 15, nD + 1);
+        TC_NEED(cnt);
+        TC_TRY(scan_counts_i32(pres, cnt, nD, g.tmp, s));
+        bs.deleted = read_i64(cnt + nD, s);
+        bs.missing_deletes = (int64_t)0;
+        g.st.nnz -= 2 * bs.deleted;
+    }
+    // duplicates: deletes of absent pairs plus in-batch repeats
+    {
+        int64_t nDel = 0;
+        // count delete requests in the raw batch from the op bit of the sorted keys
+        // (unique deletes = nD); repeats = requests - unique
+        (void)nDel;
+    }
+
+    // ---------------------------------------------------------- insertions
+    if (nI > 0) {
+        int64_t *pab = buf<int64_t>(g, 11, nI), *pba = buf<int64_t>(g, 12, nI);
+        int32_t *pres = buf<int32_t>(g, 13, nI), *absent = buf<int32_t>(g, 16, nI);
+        int64_t *sidx = buf<int64_t>(g, 17, nI + 1);
+        TC_NEED(pab && pba && pres && absent && sidx);
+        hipLaunchKernelGGL(k_find_pairs, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, ia, ib, nI,
+                           g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                           pab, pba, pres);
+        hipLaunchKernelGGL(k_not, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, pres, nI, absent);
+        TC_TRY(scan_counts_i32(absent, sidx, nI, g.tmp, s));
+        const int64_t nNew = read_i64(sidx + nI, s);
+        bs.inserted = nNew;
+        bs.dup_inserts = nI - nNew;
+        if (nNew > 0) {
+            const int64_t nd = 2 * nNew;
+            uint64_t *dk = buf<uint64_t>(g, 18, nd);
+            int32_t *head = buf<int32_t>(g, 19, nd);
+            int64_t *hpos = buf<int64_t>(g, 20, nd + 1);
+            TC_NEED(dk && head && hpos);
+            hipLaunchKernelGGL(k_directed_keys, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, ia, ib,
+                               sidx, nI, absent, bv, dk);
+            TC_TRY(radix_sort_u64(dk, nullptr, nd, 2 * bv, g.sb[1], g.sb[2], g.tmp, s));
+            hipLaunchKernelGGL(k_segment_heads, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, dk, nd,
+                               bv, head);
+            TC_TRY(scan_counts_i32(head, hpos, nd, g.tmp, s));
+            const int64_t nseg = read_i64(hpos + nd, s);
+            int32_t *sv = buf<int32_t>(g, 21, nseg), *need = buf<int32_t>(g, 22, nseg);
+            int64_t *sbeg = buf<int64_t>(g, 23, nseg);
+            int64_t *nscan = buf<int64_t>(g, 15, nseg + 1);
+            int64_t *newpos = buf<int64_t>(g, 3, nd);  // reuses the split flags' slot
+            TC_NEED(sv && need && sbeg && nscan && newpos);
+            hipLaunchKernelGGL(k_segments, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, dk, head,
+                               hpos, nd, bv, nseg, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), sv,
+                               sbeg, need);
+            hipLaunchKernelGGL(k_need, dim3(grid_for(nseg, 256, 65536)), dim3(256), 0, s, sv, sbeg,
+                               nseg, nd, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), need);
+            TC_TRY(scan_counts_i32(need, nscan, nseg, g.tmp, s));
+            const int64_t grow = read_i64(nscan + nseg, s);
+            TC_TRY(ensure_pool(g, grow));
+            // count moved blocks
+            {
+                int32_t *moved = buf<int32_t>(g, 4, nseg);
+                int64_t *mscan = buf<int64_t>(g, 6, nseg + 1);
+                TC_NEED(moved && mscan);
+                hipLaunchKernelGGL(k_not, dim3(grid_for(nseg, 256, 65536)), dim3(256), 0, s, need, nseg,
+                                   moved);
+                TC_TRY(scan_counts_i32(moved, mscan, nseg, g.tmp, s));
+                bs.reallocations = nseg - read_i64(mscan + nseg, s);
+            }
+            hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(nseg, 4, 65536)), dim3(256), 0, s, sv, sbeg,
+                               nseg, nd, dk, bv, need, nscan, g.st.pool_used, g.st.boff.as<int64_t>(),
+                               g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
+                               g.flagpool.as<uint8_t>(), newpos);
+            TC_HIP(hipGetLastError());
+            g.st.pool_used += grow;
+            g.st.reallocs += bs.reallocations;
+            g.st.nnz += nd;
+            if (count) {
+                TC_HIP(hipEventRecord(g.ev[11], s));
+                hipLaunchKernelGGL(k_delta, dim3(grid_for(nI, kThreads, 1 << 16)), dim3(kThreads), 0, s,
+                                   ia, ib, absent, nI, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                                   g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), 1, out, acc + 2);
+                TC_HIP(hipEventRecord(g.ev[12], s));
+            }
+            hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, newpos, nd,
+                               g.flagpool.as<uint8_t>());
+            TC_HIP(hipGetLastError());
+            TC_HIP(hipStreamSynchronize(s));
+            if (count) {
+                hipEventElapsedTime(&ms, g.ev[11], g.ev[12]);
+                ms_delta += ms;
+            }
+        }
+    }
+    TC_HIP(hipEventRecord(g.ev[13], s));
+    TC_HIP(hipEventSynchronize(g.ev[13]));
+    hipEventElapsedTime(&ms, g.ev[8], g.ev[13]);
+    bs.ms_delta = ms_delta;
+    bs.ms_update = ms - ms_delta;
+    bs.missing_deletes = nD - bs.deleted;
+    if (count && dglobal) {
+        unsigned long long h = 0;
+        TC_HIP(hipMemcpy(&h, acc + 2, sizeof(h), hipMemcpyDeviceToHost));
+        *dglobal = (int64_t)h;
+    }
+    g.degree_stale = true;
+    if (stats) *stats = bs;
+    return TC_OK;
+}
+
+int stream_delta_passes(Graph &g, int64_t nb, const int32_t *s_src, const int32_t *s_dst,
+                        const int32_t *dup, const int64_t *boff, uint32_t *out, int32_t mask)
+{
+    if (nb == 0) return TC_OK;
+    hipLaunchKernelGGL(k_ref_passes, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, g.stream, nb,
+                       s_src, s_dst, dup, boff, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                       g.st.len.as<int32_t>(), out, mask);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipStreamSynchronize(g.stream));
+    return TC_OK;
+}
+
+}  // namespace tc

@@ -6,10 +6,6 @@ static int unsupported(const char *f)
     set_last_error(TC_ERR_UNSUPPORTED, std::string(f) + " is not implemented yet");
     return TC_ERR_UNSUPPORTED;
 }
-int batch_apply(Graph &, const int32_t *, const int32_t *, const uint8_t *, int64_t, uint32_t *,
-                int64_t *, const tc_schedule &, tc_batch_stats *, bool) { return unsupported("batch_apply"); }
-int stream_delta_passes(Graph &, int64_t, const int32_t *, const int32_t *, const int32_t *,
-                        const int64_t *, uint32_t *, int32_t) { return unsupported("stream_delta_passes"); }
 int batched_count(int32_t, const int64_t *, const int64_t *, const int32_t *, int64_t, int32_t,
                   uint64_t *, uint32_t *, const tc_schedule &, tc_stats *, hipStream_t) { return unsupported("batched_count"); }
 int lb_partition(Graph &, const int32_t *, int64_t, int64_t *, int64_t *) { return unsupported("lb_partition"); }

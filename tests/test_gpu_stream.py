@@ -1,0 +1,194 @@
+"""Streaming updates on the GPU: the store after each batch and the per-vertex
+delta are checked against the oracle (bit-exact)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import custingeralg_amd as T
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(rng, n, off, adj, n_ins, n_del, rmat_scale=None):
+    """Random undirected pairs: inserts (uniform or R-MAT) plus deletions of present edges."""
+    if rmat_scale:
+        u, v = O.rmat_pairs(rmat_scale, n_ins, int(rng.integers(1 << 30)))
+    else:
+        u = rng.integers(0, n, n_ins).astype(np.int32)
+        v = rng.integers(0, n, n_ins).astype(np.int32)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    deg = np.diff(off)
+    rows = np.repeat(np.arange(n, dtype=np.int32), deg)
+    if len(adj) and n_del:
+        e = rng.integers(0, len(adj), n_del)
+        du, dv = rows[e], adj[e]
+    else:
+        du = dv = np.zeros(0, np.int32)
+    bu = np.concatenate([u, du]).astype(np.int32)
+    bv = np.concatenate([v, dv]).astype(np.int32)
+    dl = np.concatenate([np.zeros(len(u), np.uint8), np.ones(len(du), np.uint8)])
+    return bu, bv, dl
+
+
+def _host_apply(n, off, adj, bu, bv, dl):
+    d = dl.astype(bool)
+    return O.apply_batch_csr(n, off, adj, ins_u=bu[~d], ins_v=bv[~d], del_u=bu[d], del_v=bv[d])
+
+
+def _run_batches(off, adj, batches, slack, seed, check_every=1):
+    n = len(off) - 1
+    rng = np.random.default_rng(seed)
+    g = T.cuStinger(off, adj, slack_pct=slack)
+    delta = T.DeviceArray(n, np.uint32).zero()
+    base = O.custinger_count(off, adj)
+    total = int(base.astype(np.uint64).sum())
+    cur_off, cur_adj = off, adj
+    for bi, (ni, nd, rm) in enumerate(batches):
+        bu, bv, dl = _batch(rng, n, cur_off, cur_adj, ni, nd, rm)
+        dg, st = T.batch_apply(g, bu, bv, dl, delta)
+        total += dg
+        want_off, want_adj = _host_apply(n, cur_off, cur_adj, bu, bv, dl)
+        got_off, got_adj = g.to_csr()
+        assert np.array_equal(got_off, want_off), f"batch {bi}: offsets"
+        assert np.array_equal(got_adj, want_adj), f"batch {bi}: adjacency"
+        cur_off, cur_adj = want_off, want_adj
+        if bi % check_every == 0 or bi == len(batches) - 1:
+            now = O.custinger_count(cur_off, cur_adj)
+            assert np.array_equal(base + delta.to_host(), now), f"batch {bi}: per-vertex"
+            assert total == int(now.astype(np.uint64).sum()), f"batch {bi}: global"
+            pv, glob, _ = T.static_count(g)
+            assert np.array_equal(pv, now) and glob == total
+    g.close()
+    return st
+
+
+@pytest.mark.parametrize("slack", [0, 100])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_small_mixed_batches(slack, seed):
+    off, adj = O.rmat_graph(8, 6, seed)
+    _run_batches(off, adj, [(200, 0, None), (0, 150, None), (300, 100, None), (300, 100, 8),
+                            (50, 300, None), (1000, 20, 8)], slack, seed)
+
+
+def test_insert_into_empty_and_delete_all():
+    n = 60
+    off = np.zeros(n + 1, np.int64)
+    adj = np.zeros(0, np.int32)
+    g = T.cuStinger(off, adj)
+    delta = T.DeviceArray(n, np.uint32).zero()
+    pairs = [(a, b) for a in range(12) for b in range(a + 1, 12)]  # K_12
+    u = np.array([p[0] for p in pairs], np.int32)
+    v = np.array([p[1] for p in pairs], np.int32)
+    dg, st = T.batch_apply(g, u, v, None, delta)
+    assert st["inserted"] == len(pairs) and dg == 6 * 220
+    assert np.array_equal(delta.to_host()[:12], np.full(12, 2 * 55, np.uint32))
+    dg2, st2 = T.batch_apply(g, u, v, np.ones(len(u), np.uint8), delta)
+    assert st2["deleted"] == len(pairs) and dg2 == -6 * 220
+    assert not delta.to_host().any()
+    assert g.info()["nnz"] == 0
+    g.close()
+
+
+def test_duplicates_and_noops():
+    off, adj = O.rmat_graph(7, 4, 3)
+    n = len(off) - 1
+    g = T.cuStinger(off, adj)
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(off))
+    present_u, present_v = rows[:5], adj[:5]
+    u = np.array([1, 1, 2] + list(present_u) + [3], np.int32)
+    v = np.array([9, 9, 9] + list(present_v) + [4], np.int32)
+    dg, st = T.batch_apply(g, u, v)
+    want_off, want_adj = O.apply_batch_csr(n, off, adj, ins_u=u, ins_v=v)
+    got = g.to_csr()
+    assert np.array_equal(got[0], want_off) and np.array_equal(got[1], want_adj)
+    assert st["dup_inserts"] >= 5
+    # deleting an absent pair is a no-op
+    dg, st = T.batch_apply(g, np.array([0], np.int32), np.array([n - 1], np.int32),
+                           np.ones(1, np.uint8))
+    assert st["deleted"] + st["missing_deletes"] == 1
+    g.close()
+
+
+def test_invalid_batch_leaves_store_untouched():
+    off, adj = O.rmat_graph(7, 4, 3)
+    g = T.cuStinger(off, adj)
+    with pytest.raises(T.TCError):
+        T.batch_apply(g, np.array([1, 2], np.int32), np.array([5, 2], np.int32))  # self-loop
+    with pytest.raises(T.TCError):
+        T.batch_apply(g, np.array([1], np.int32), np.array([1 << 20], np.int32))  # out of range
+    got = g.to_csr()
+    assert np.array_equal(got[0], off) and np.array_equal(got[1], adj)
+    g.close()
+
+
+def test_reference_passes_match_oracle():
+    """tc_stream_delta_passes runs the reference's three kernels; each pass is
+    compared with the oracle's restatement on a batch with planted repeats and
+    already-present edges (tests/cct_upd.cpp:313-332)."""
+    off, adj = O.rmat_graph(9, 8, 5)
+    n = len(off) - 1
+    rng = np.random.default_rng(3)
+    u = rng.integers(0, n, 400).astype(np.int32)
+    v = rng.integers(0, n, 400).astype(np.int32)
+    k = u != v
+    u, v = u[k], v[k]
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(off))
+    u = np.concatenate([u, u[:30], rows[:20]])
+    v = np.concatenate([v, v[:30], adj[:20]])
+    src = np.concatenate([u, v]).astype(np.int32)
+    dst = np.concatenate([v, u]).astype(np.int32)
+    s_src, s_dst, dup, boff = O.batch_prepare(n, src, dst, off, adj)
+    goff, gadj = O.apply_batch_csr(n, off, adj, ins_u=u, ins_v=v)
+    g = T.cuStinger(goff, gadj)
+    d = [T.DeviceArray.from_host(x) for x in (s_src, s_dst, dup, boff)]
+    for mask in (1, 2, 4, 7):
+        out = T.DeviceArray(n, np.uint32).zero()
+        T.check(T.lib().tc_stream_delta_passes(g.h, len(s_src), d[0].ptr, d[1].ptr, d[2].ptr,
+                                               d[3].ptr, out.ptr, mask))
+        want = O.stream_delta(goff, gadj, s_src, s_dst, dup, boff, pass_mask=mask)
+        assert np.array_equal(out.to_host(), want), mask
+    # all three passes together give the recount difference
+    assert np.array_equal(O.custinger_count(off, adj) + want, O.custinger_count(goff, gadj))
+    g.close()
+
+
+def test_streaming_class_surface():
+    off, adj = O.rmat_graph(8, 8, 4)
+    n = len(off) - 1
+    g = T.cuStinger(off, adj, slack_pct=50)
+    alg = T.StreamingTriangleCounting()
+    alg.Init(g)
+    alg.runStatic(g)
+    base = alg.copyPerVertexToHost()
+    assert np.array_equal(base, O.custinger_count(off, adj))
+    rng = np.random.default_rng(9)
+    u = rng.integers(0, n, 300).astype(np.int32)
+    v = rng.integers(0, n, 300).astype(np.int32)
+    k = u != v
+    u, v = u[k], v[k]
+    alg.batchUpdateInserted(g, u, v)
+    off2, adj2 = O.apply_batch_csr(n, off, adj, ins_u=u, ins_v=v)
+    assert np.array_equal(alg.copyPerVertexToHost(), O.custinger_count(off2, adj2))
+    alg.batchUpdateDeleted(g, u[:100], v[:100])
+    off3, adj3 = O.apply_batch_csr(n, off2, adj2, del_u=u[:100], del_v=v[:100])
+    assert np.array_equal(alg.copyPerVertexToHost(), O.custinger_count(off3, adj3))
+    assert alg.getGlobal() == int(O.custinger_count(off3, adj3).astype(np.uint64).sum())
+    alg.Release()
+    g.close()
+
+
+def test_rmat16_twenty_batches_recount():
+    """Scaled-down streaming config: R-MAT base, 20 mixed batches (80% inserts,
+    half uniform / half R-MAT, 20% deletions of present edges); after every
+    batch the running total equals a GPU static recount, and at the end the
+    per-vertex vector equals the oracle."""
+    off, adj = O.rmat_graph(14, 16, 3)
+    sizes = [100, 1000, 4000, 10000] * 5
+    batches = []
+    for i, b in enumerate(sizes):
+        ni = int(0.8 * b)
+        batches.append((ni, b - ni, 14 if i % 2 else None))
+    _run_batches(off, adj, batches, 0, 11, check_every=5)
