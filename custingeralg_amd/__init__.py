@@ -24,7 +24,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcustinger_tc.so")
+LIB_PATH = os.environ.get("CUSTINGER_TC_LIB") or os.path.join(_HERE, "libcustinger_tc.so")
 
 TC_OK, TC_ERR_INVALID, TC_ERR_OOM, TC_ERR_HIP, TC_ERR_UNSUPPORTED, TC_ERR_CAPACITY = range(6)
 TC_HOST, TC_DEVICE = 0, 1
@@ -43,7 +43,7 @@ EXPORTS = [
     "tc_graph_to_csr", "tc_graph_degrees", "tc_schedule_default", "tc_static_count",
     "tc_csr_count_device", "tc_batch_apply", "tc_batch_update_only", "tc_stream_delta_passes",
     "tc_batched_count", "tc_lb_partition", "tc_lb_degree_bins", "tc_gen_rmat_pairs",
-    "tc_gen_er_pairs", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
+    "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune",
 ]
 
@@ -57,7 +57,8 @@ class TCError(RuntimeError):
 class Schedule(C.Structure):
     _fields_ = [("method", C.c_int32), ("policy", C.c_int32), ("small_max", C.c_int32),
                 ("mid_max", C.c_int32), ("job_items", C.c_int32), ("ratio_x16", C.c_int32),
-                ("persistent", C.c_int32), ("reserved", C.c_int32 * 9)]
+                ("persistent", C.c_int32), ("flush_items", C.c_int32),
+                ("reserved", C.c_int32 * 8)]
 
 
 class Stats(C.Structure):
@@ -131,6 +132,8 @@ def lib():
         L.tc_lb_degree_bins.argtypes = [vp, vp, i64, C.POINTER(i64)]
         L.tc_gen_rmat_pairs.argtypes = [i32, i64, C.c_double, C.c_double, C.c_double, u64, vp, vp, vp]
         L.tc_gen_er_pairs.argtypes = [i32, i64, u64, vp, vp, vp]
+        L.tc_gen_rmat_multi.argtypes = [i32, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp,
+                                        vp, vp]
         L.tc_canon_csr_device.argtypes = [i32, i64, vp, vp, C.POINTER(vp), C.POINTER(vp),
                                           C.POINTER(i64), vp]
         L.tc_collect_timings.argtypes = [vp, i64, u64, vp, vp, C.POINTER(i64)]
@@ -324,6 +327,48 @@ def batch_apply(g: cuStinger, u, v, is_delete=None, delta: DeviceArray | None = 
     return 0, st.as_dict()
 
 
+def lb_partition(g: cuStinger, verts) -> np.ndarray:
+    """cusLoadBalance::LoadBalanceArray (include/load_balance.cuh:77-102) on the device."""
+    dv = DeviceArray.from_host(np.ascontiguousarray(verts, np.int32))
+    parts = DeviceArray(1000, np.int64)
+    P = C.c_int64(0)
+    check(lib().tc_lb_partition(g.h, dv.ptr, dv.n, parts.ptr, C.byref(P)))
+    return parts.to_host()[:P.value]
+
+
+def lb_degree_bins(g: cuStinger, verts) -> np.ndarray:
+    """Vertices per degree bin (>=64, 16..63, <16) as in include/operators.cuh:164-288."""
+    dv = DeviceArray.from_host(np.ascontiguousarray(verts, np.int32))
+    out = (C.c_int64 * 3)()
+    check(lib().tc_lb_degree_bins(g.h, dv.ptr, dv.n, out))
+    return np.array(out[:], np.int64)
+
+
+def batched_count(voff, off, adj, schedule: Schedule | None = None, per_vertex=False):
+    """Per-graph global counts of G graphs concatenated with global vertex ids."""
+    voff = np.ascontiguousarray(voff, np.int64)
+    G = len(voff) - 1
+    dvo, doff, dadj = (DeviceArray.from_host(np.ascontiguousarray(x, t))
+                       for x, t in ((voff, np.int64), (off, np.int64), (adj, np.int32)))
+    pg = DeviceArray(G, np.uint64)
+    nv = len(off) - 1
+    pv = DeviceArray(nv, np.uint32) if per_vertex else None
+    st = Stats()
+    sp = C.byref(schedule) if schedule is not None else None
+    check(lib().tc_batched_count(G, dvo.ptr, doff.ptr, dadj.ptr, dadj.n, nv, pg.ptr,
+                                 pv.ptr if pv else None, sp, C.byref(st), None))
+    return pg.to_host(), (pv.to_host() if pv else None), st.as_dict()
+
+
+def batched_count_device(voff: DeviceArray, off: DeviceArray, adj: DeviceArray,
+                         per_graph: DeviceArray, schedule: Schedule | None = None):
+    st = Stats()
+    sp = C.byref(schedule) if schedule is not None else None
+    check(lib().tc_batched_count(voff.n - 1, voff.ptr, off.ptr, adj.ptr, adj.n, off.n - 1,
+                                 per_graph.ptr, None, sp, C.byref(st), None))
+    return st.as_dict()
+
+
 # ------------------------------------------------------------ reference shape
 class StaticTriangleCounting:
     """StaticAlgorithm surface (include/algs.cuh:20-29) for triangle counting."""
@@ -434,6 +479,22 @@ def er_graph_device(n: int, avg_deg: int, seed: int, slack_pct=0) -> cuStinger:
     du, dv = DeviceArray(m, np.int32), DeviceArray(m, np.int32)
     check(lib().tc_gen_er_pairs(n, m, seed, du.ptr, dv.ptr, None))
     return _canon(n, m, du, dv, slack_pct)
+
+
+def batched_graph_device(n, ef, seeds, abc=(0.57, 0.19, 0.19)):
+    """Concatenated CSR of G R-MAT graphs built on the device -> (voff, off, adj) DeviceArrays."""
+    n = np.ascontiguousarray(n, np.int32)
+    ef = np.ascontiguousarray(ef, np.int32)
+    seeds = np.ascontiguousarray(seeds, np.uint64)
+    m = int((n.astype(np.int64) * ef).sum())
+    du, dv = DeviceArray(m, np.int32), DeviceArray(m, np.int32)
+    check(lib().tc_gen_rmat_multi(len(n), n.ctypes.data, ef.ctypes.data, seeds.ctypes.data,
+                                  abc[0], abc[1], abc[2], du.ptr, dv.ptr, None))
+    voff = np.concatenate([[0], np.cumsum(n.astype(np.int64))])
+    doff, dadj = canon_csr_device(int(voff[-1]), du, dv)
+    du.free()
+    dv.free()
+    return DeviceArray.from_host(voff), doff, dadj
 
 
 def canon_csr_device(n: int, du: DeviceArray, dv: DeviceArray):

@@ -26,6 +26,8 @@ int lb_degree_bins(Graph &g, const int32_t *verts, int64_t len, int64_t counts[3
 int gen_rmat(int32_t scale, int64_t m, double a, double b, double c, uint64_t seed, int32_t *u,
              int32_t *v, hipStream_t s);
 int gen_er(int32_t n, int64_t m, uint64_t seed, int32_t *u, int32_t *v, hipStream_t s);
+int gen_rmat_multi(int32_t G, const int32_t *n, const int32_t *ef, const uint64_t *seeds, double a,
+                   double b, double c, int32_t *u, int32_t *v, hipStream_t s);
 int canon_csr_device(int32_t nv, int64_t m, const int32_t *u, const int32_t *v, int64_t **off,
                      int32_t **adj, int64_t *nnz, hipStream_t s);
 int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, float *cost,
@@ -347,6 +349,17 @@ int tc_gen_er_pairs(int32_t n, int64_t m, uint64_t seed, int32_t *u_dev, int32_t
     TC_GUARD({
         TC_CHECK_ARG(n >= 1 && m >= 0 && (m == 0 || (u_dev && v_dev)), "tc_gen_er_pairs: bad arguments");
         return tc::gen_er(n, m, seed, u_dev, v_dev, (hipStream_t)hip_stream);
+    })
+}
+
+int tc_gen_rmat_multi(int32_t G, const int32_t *n, const int32_t *ef, const uint64_t *seeds,
+                      double a, double b, double c, int32_t *u_dev, int32_t *v_dev, void *hip_stream)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(G >= 0 && (G == 0 || (n && ef && seeds && u_dev && v_dev)) && a >= 0 && b >= 0 &&
+                         c >= 0 && a + b + c <= 1.0,
+                     "tc_gen_rmat_multi: bad arguments");
+        return tc::gen_rmat_multi(G, n, ef, seeds, a, b, c, u_dev, v_dev, (hipStream_t)hip_stream);
     })
 }
 

@@ -62,12 +62,13 @@ __global__ void k_gen_er(int32_t n, int64_t m, uint64_t key, int32_t *__restrict
 // Both directions of every non-loop pair; loops become the all-ones key,
 // which sorts after every valid key.
 __global__ void k_pair_keys(const int32_t *__restrict__ u, const int32_t *__restrict__ v, int64_t m,
-                            int bits_v, uint64_t invalid, uint64_t *__restrict__ keys)
+                            int32_t nv, int bits_v, uint64_t invalid, uint64_t *__restrict__ keys)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
          i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t a = (uint32_t)u[i], b = (uint32_t)v[i];
-        bool loop = a == b;
+        // self-loops and pairs with an endpoint outside [0,nv) are dropped
+        bool loop = a == b || a >= (uint64_t)nv || b >= (uint64_t)nv;
         keys[2 * i] = loop ? invalid : ((a << bits_v) | b);
         keys[2 * i + 1] = loop ? invalid : ((b << bits_v) | a);
     }
@@ -123,6 +124,42 @@ __global__ void k_fill_off(int64_t *off, int32_t nv)
 
 uint64_t host_key(uint64_t seed) { return mix64(seed + 0x9E3779B97F4A7C15ULL); }
 
+struct MultiGraph {
+    int32_t scale;
+    int32_t n;
+    int64_t voff;   // first global vertex id
+    int64_t poff;   // first pair index
+    uint64_t key;
+};
+
+// Pair p belongs to the graph whose pair range holds it; rejected pairs
+// (an endpoint >= n) become the self-loop (voff, voff).
+__global__ void k_gen_rmat_multi(const MultiGraph *__restrict__ gs, int32_t G, int64_t m,
+                                 uint64_t t0, uint64_t t1, uint64_t t2, int32_t *__restrict__ u,
+                                 int32_t *__restrict__ v)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < m;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (gs[mid].poff <= p) lo = mid; else hi = mid - 1;
+        }
+        const MultiGraph g = gs[lo];
+        const uint64_t i = (uint64_t)(p - g.poff);
+        uint32_t x = 0, y = 0;
+        for (int l = 0; l < g.scale; ++l) {
+            const uint64_t r = rng_keyed(g.key, i * 64 + (uint64_t)l) >> 11;
+            const uint32_t bit = 1u << (g.scale - 1 - l);
+            x |= (r >= t1) ? bit : 0u;
+            y |= ((r >= t0 && r < t1) || (r >= t2)) ? bit : 0u;
+        }
+        const bool ok = (int32_t)x < g.n && (int32_t)y < g.n;
+        u[p] = (int32_t)(g.voff + (ok ? x : 0));
+        v[p] = (int32_t)(g.voff + (ok ? y : 0));
+    }
+}
+
 }  // namespace
 
 int gen_rmat(int32_t scale, int64_t m, double a, double b, double c, uint64_t seed, int32_t *u,
@@ -136,6 +173,45 @@ int gen_rmat(int32_t scale, int64_t m, double a, double b, double c, uint64_t se
                        t1, t2, host_key(seed), u, v);
     TC_HIP(hipGetLastError());
     TC_HIP(hipStreamSynchronize(s));
+    return TC_OK;
+}
+
+int gen_rmat_multi(int32_t G, const int32_t *n, const int32_t *ef, const uint64_t *seeds, double a,
+                   double b, double c, int32_t *u, int32_t *v, hipStream_t s)
+{
+    std::vector<MultiGraph> gs((size_t)G);
+    int64_t voff = 0, poff = 0;
+    for (int32_t i = 0; i < G; ++i) {
+        if (n[i] < 1 || ef[i] < 0) {
+            set_last_error(TC_ERR_INVALID, "tc_gen_rmat_multi: n must be >= 1 and ef >= 0");
+            return TC_ERR_INVALID;
+        }
+        int sc = 0;
+        while ((1 << sc) < n[i]) ++sc;
+        gs[i] = MultiGraph{sc < 1 ? 1 : sc, n[i], voff, poff, host_key(seeds[i])};
+        voff += n[i];
+        poff += (int64_t)ef[i] * n[i];
+    }
+    if (voff > INT32_MAX) {
+        set_last_error(TC_ERR_CAPACITY, "tc_gen_rmat_multi: more than 2^31-1 vertices");
+        return TC_ERR_CAPACITY;
+    }
+    if (poff == 0) return TC_OK;
+    const double two53 = 9007199254740992.0;
+    uint64_t t0 = (uint64_t)(a * two53), t1 = (uint64_t)((a + b) * two53),
+             t2 = (uint64_t)((a + b + c) * two53);
+    DevBuf dg;
+    TC_TRY(dg.ensure(gs.size() * sizeof(MultiGraph)));
+    TC_HIP(hipMemcpyAsync(dg.p, gs.data(), gs.size() * sizeof(MultiGraph), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gen_rmat_multi, dim3(grid_for(poff, 256, 1 << 16)), dim3(256), 0, s,
+                       dg.as<MultiGraph>(), G, poff, t0, t1, t2, u, v);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dg.release();
+    if (e != hipSuccess) {
+        set_last_error(TC_ERR_HIP, std::string("tc_gen_rmat_multi: ") + hipGetErrorString(e));
+        return TC_ERR_HIP;
+    }
     return TC_OK;
 }
 
@@ -178,7 +254,7 @@ int canon_csr_device(int32_t nv, int64_t m, const int32_t *u, const int32_t *v, 
     if (n > 0) {
         if ((st = keys.ensure((size_t)n * sizeof(uint64_t))) != TC_OK) goto fail;
         hipLaunchKernelGGL(k_pair_keys, dim3(grid_for(m, 256, 1 << 16)), dim3(256), 0, s, u, v, m,
-                           bits_v, invalid, keys.as<uint64_t>());
+                           nv, bits_v, invalid, keys.as<uint64_t>());
         if ((st = radix_sort_u64(keys.as<uint64_t>(), nullptr, n, bits, alt_keys, alt_vals, tmp, s)) !=
             TC_OK)
             goto fail;

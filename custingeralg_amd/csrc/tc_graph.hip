@@ -364,10 +364,48 @@ int graph_to_csr(Graph &g, int64_t *off, int32_t *adj, int location)
     return TC_OK;
 }
 
+// A read-only view of a caller's device CSR: per-vertex offsets and lengths
+// are built, the adjacency array itself is used in place and never freed.
+int graph_borrow_csr(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_t *d_adj,
+                     hipStream_t s, Graph **out)
+{
+    Graph *g = new Graph();
+    g->st.nv = nv;
+    g->st.nnz = nnz;
+    g->stream = s;
+    g->borrowed = true;
+    const size_t nvb = (size_t)std::max<int32_t>(nv, 1);
+    int st;
+    if ((st = g->st.boff.ensure(nvb * sizeof(int64_t))) != TC_OK ||
+        (st = g->st.len.ensure(nvb * sizeof(int32_t))) != TC_OK ||
+        (st = g->st.cap.ensure(nvb * sizeof(int32_t))) != TC_OK) {
+        graph_destroy(g);
+        return st;
+    }
+    if (nv > 0)
+        hipLaunchKernelGGL(k_len_from_off, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, d_off, nv,
+                           g->st.len.as<int32_t>(), g->st.boff.as<int64_t>(), g->st.cap.as<int32_t>(),
+                           0);
+    g->st.pool.p = const_cast<int32_t *>(d_adj);
+    g->st.pool.bytes = (size_t)nnz * sizeof(int32_t);
+    g->st.pool_used = nnz;
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
+        set_last_error(TC_ERR_HIP, "borrowed graph setup failed");
+        graph_destroy(g);
+        return TC_ERR_HIP;
+    }
+    *out = g;
+    return TC_OK;
+}
+
 void graph_destroy(Graph *g)
 {
     if (!g) return;
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
+    if (g->borrowed) {
+        g->st.pool.p = nullptr;
+        g->st.pool.bytes = 0;
+    }
     DevBuf *bufs[] = {&g->st.boff, &g->st.len, &g->st.cap, &g->st.pool, &g->vinfo, &g->padj,
                       &g->coff, &g->crow, &g->work, &g->jobs, &g->tmp, &g->tmp2, &g->tmp3, &g->tmp4,
                       &g->accum, &g->bsrc, &g->bdst, &g->bflag, &g->bkeys, &g->bkeys2,

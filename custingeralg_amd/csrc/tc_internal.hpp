@@ -118,6 +118,12 @@ struct Graph {
     DevBuf flagpool, vmark;
     DevBuf sb[24];
     bool degree_stale = false;
+    // batched counting: graph boundaries and per-graph output (device)
+    const int64_t *batch_voff = nullptr;
+    int32_t batch_ngraphs = 0;
+    unsigned long long *batch_per_graph = nullptr;
+    // the pool is the caller's adjacency array (read-only, not freed)
+    bool borrowed = false;
     hipEvent_t ev[16] = {};
     bool events_ready = false;
     int ensure_events();
@@ -133,6 +139,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
 int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t *adj, int location,
                      uint32_t flags, int32_t slack_pct, Graph **out);
 int graph_to_csr(Graph &g, int64_t *off, int32_t *adj, int location);
+int graph_borrow_csr(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_t *d_adj,
+                     hipStream_t s, Graph **out);
 void graph_destroy(Graph *g);
 tc_schedule default_schedule();
 

@@ -32,6 +32,10 @@ namespace tc {
 enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2 };
 constexpr int kUnroll = 4;
 constexpr int kBigTask = 64;  // hash tasks at least this long get the whole wave
+#ifndef TC_BIG_UNROLL
+#define TC_BIG_UNROLL 8
+#endif
+constexpr int kBigUnroll = TC_BIG_UNROLL;  // loads in flight per lane on long tasks
 
 __device__ inline int64_t readlane_i64(int64_t x, int lane)
 {
@@ -309,7 +313,8 @@ struct PlanParams {
     int32_t lmax[3];     // owner-list capacity of each bin's kernel
     int32_t nw[3];       // wavefronts per block of each bin's kernel
     int32_t bin_max[2];  // |owner| <= bin_max[0] -> bin 0, <= bin_max[1] -> bin 1, else 2
-    int32_t job_items;   // target estimated items per job
+    int32_t job_items;   // target estimated items per job (per wavefront)
+    int32_t flush_items; // minimum items per owner slot in a job
 };
 
 __device__ inline int32_t est_items(int32_t L, int32_t pl)
@@ -337,13 +342,13 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nchunks = ceil_div(nnz, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    // per-lane bin statistics, reduced once at the end
+    unsigned long long st_t[3] = {0, 0, 0}, st_b[3] = {0, 0, 0};
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         int64_t e = c * kWave + lane;
         int32_t b = wave_row_of(coff, crow, nv, c, nnz);
-        unsigned long long w = 0, bytes = 0;
-        int bin = 0;
-        bool task = false;
+        uint32_t w = 0;
         if (e < nnz) {
             VInfo vb = vi[b];
             int64_t k = e - coff[b];
@@ -355,33 +360,38 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
                 int32_t pl = ORIENTED ? va.len - va.dm : va.len;
                 bool ok = ORIENTED ? (pl > 0) : (a != b && va.len >= 2 && vb.len >= 2);
                 if (ok) {
-                    task = true;
-                    w = (unsigned long long)est_items(min(L, P.lmax[2]), pl);
-                    bytes = 4ull * (unsigned long long)(L + pl);
-                    bin = owner_bin(L, P);
+                    w = (uint32_t)est_items(min(L, P.lmax[2]), pl);
+                    const int bin = owner_bin(L, P);
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+                        if (bin == q) {
+                            st_t[q] += 1;
+                            st_b[q] += 4ull * (unsigned long long)(L + pl);
+                        }
                 }
             }
         }
-        // segmented (by owner row) inclusive scan of w across the wave
-        unsigned long long x = w;
+        // segmented (by owner row) inclusive scan of w across the wave; a
+        // chunk's sum fits 32 bits (each item estimate is below 2^25)
+        uint32_t x = w;
 #pragma unroll
         for (int d = 1; d < kWave; d <<= 1) {
-            unsigned long long y = __shfl_up(x, d, kWave);
+            uint32_t y = __shfl_up(x, d, kWave);
             int32_t rb = __shfl_up(b, d, kWave);
             if (lane >= d && rb == b) x += y;
         }
         int32_t bnext = __shfl_down(b, 1, kWave);
         bool tail = (lane == kWave - 1) || (bnext != b) || (e + 1 >= nnz);
-        if (e < nnz && tail && x) atomicAdd(&work[b], x);
-        if (bstats) {
+        if (e < nnz && tail && x) atomicAdd(&work[b], (unsigned long long)x);
+    }
+    if (bstats) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                unsigned long long t = wave_sum<unsigned long long>((task && bin == q) ? 1ull : 0ull);
-                unsigned long long by = wave_sum<unsigned long long>((task && bin == q) ? bytes : 0ull);
-                if (lane == 0 && t) {
-                    atomicAdd(&bstats[2 * q], t);
-                    atomicAdd(&bstats[2 * q + 1], by);
-                }
+        for (int q = 0; q < 3; ++q) {
+            unsigned long long t = wave_sum<unsigned long long>(st_t[q]);
+            unsigned long long by = wave_sum<unsigned long long>(st_b[q]);
+            if (lane == 0 && t) {
+                atomicAdd(&bstats[2 * q], t);
+                atomicAdd(&bstats[2 * q + 1], by);
             }
         }
     }
@@ -403,7 +413,10 @@ __global__ void k_plan_jobs(const VInfo *__restrict__ vi, const unsigned long lo
             int64_t tiles = ceil_div(L, P.lmax[bin]);
             int64_t per_tile = (int64_t)(w / (unsigned long long)tiles) + 1;
             // a job amortises building its LDS table over at least 16 items per slot
-            int64_t grain = max<int64_t>((int64_t)P.job_items * P.nw[bin], 16 * (int64_t)min(L, P.lmax[bin]));
+            // Each job ends by flushing up to one global atomic per owner slot,
+            // so a job gets at least flush_items items per slot.
+            int64_t grain = max<int64_t>((int64_t)P.job_items * P.nw[bin],
+                                         (int64_t)P.flush_items * min(L, P.lmax[bin]));
             int64_t split = ceil_div(per_tile, grain);
             if (split > T) split = T;
             if (split < 1) split = 1;
@@ -460,6 +473,11 @@ struct CountArgs {
     int32_t policy;
     int32_t ratio_x16;
     int32_t skip_lower;  // CSR convention: entries with dest < src are skipped
+    // batched graphs: graph g owns vertices [gvoff[g], gvoff[g+1]); a job's
+    // hits are added to per_graph[g] of its owner (triangles never span graphs)
+    const int64_t *gvoff;
+    int32_t ngraphs;
+    unsigned long long *per_graph;
 };
 
 __device__ inline int lg_bucket(int32_t x)
@@ -581,15 +599,15 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                 const int32_t *lst = A.adj + readlane_i64(pb, jb2);
                 const int32_t n = __builtin_amdgcn_readlane(pl, jb2);
                 uint32_t h = 0;
-                for (int32_t k0 = lane; k0 < n; k0 += kUnroll * kWave) {
-                    int32_t w[kUnroll];
+                for (int32_t k0 = lane; k0 < n; k0 += kBigUnroll * kWave) {
+                    int32_t w[kBigUnroll];
 #pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
+                    for (int u = 0; u < kBigUnroll; ++u) {
                         const int32_t k = k0 + u * kWave;
                         w[u] = (k < n) ? lst[k] : -1;
                     }
 #pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
+                    for (int u = 0; u < kBigUnroll; ++u) {
                         const int32_t slot = T.find(w[u]);
                         if (slot >= 0) {
                             ++h;
@@ -747,6 +765,14 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         if (threadIdx.x == 0 && hjob)
             atomicAdd(&A.out[J.owner], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
         hits_all += (threadIdx.x == 0) ? hjob : 0ull;
+        if (A.per_graph && threadIdx.x == 0 && hjob) {
+            int32_t lo = 0, hi = A.ngraphs - 1;  // last graph whose first vertex <= owner
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) >> 1;
+                if (A.gvoff[mid] <= J.owner) lo = mid; else hi = mid - 1;
+            }
+            atomicAdd(&A.per_graph[lo], (ORIENTED ? 6ull : 1ull) * hjob);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0 && hits_all) atomicAdd(A.global, (ORIENTED ? 6ull : 1ull) * hits_all);
@@ -771,6 +797,7 @@ tc_schedule default_schedule()
     s.job_items = 8192;
     s.ratio_x16 = 64;  // merge when long <= 4x short (and short lists)
     s.persistent = 1;
+    s.flush_items = 1024;
     return s;
 }
 
@@ -924,6 +951,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     P.bin_max[0] = std::min(std::max(sched.small_max, 1), kL0);
     P.bin_max[1] = std::min(std::max(sched.mid_max, P.bin_max[0]), kL1);
     P.job_items = std::max(sched.job_items, 256);
+    P.flush_items = sched.flush_items > 0 ? sched.flush_items : 1024;
     TC_TRY(g.work.ensure((size_t)nv * sizeof(unsigned long long)));
     unsigned long long *work = g.work.as<unsigned long long>();
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, work, (int64_t)nv);
@@ -974,6 +1002,9 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.policy = sched.policy;
     A.ratio_x16 = sched.ratio_x16 > 0 ? sched.ratio_x16 : 64;
     A.skip_lower = skip_lower ? 1 : 0;
+    A.gvoff = g.batch_voff;
+    A.ngraphs = g.batch_ngraphs;
+    A.per_graph = g.batch_per_graph;
     A.table = nullptr;
     A.table_dim = 0;
     if (sched.policy == TC_POLICY_LEARNED) {

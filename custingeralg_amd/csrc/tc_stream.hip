@@ -629,6 +629,10 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         if (stats) *stats = bs;
         return TC_OK;
     }
+    if (g.borrowed) {
+        set_last_error(TC_ERR_UNSUPPORTED, "a borrowed (read-only) CSR view cannot be updated");
+        return TC_ERR_UNSUPPORTED;
+    }
     if (g.symmetric < 0 || g.self_loops < 0) {
         int64_t bad = 0;
         TC_TRY(validate_graph(g, true, &bad));

@@ -6,10 +6,6 @@ static int unsupported(const char *f)
     set_last_error(TC_ERR_UNSUPPORTED, std::string(f) + " is not implemented yet");
     return TC_ERR_UNSUPPORTED;
 }
-int batched_count(int32_t, const int64_t *, const int64_t *, const int32_t *, int64_t, int32_t,
-                  uint64_t *, uint32_t *, const tc_schedule &, tc_stats *, hipStream_t) { return unsupported("batched_count"); }
-int lb_partition(Graph &, const int32_t *, int64_t, int64_t *, int64_t *) { return unsupported("lb_partition"); }
-int lb_degree_bins(Graph &, const int32_t *, int64_t, int64_t *) { return unsupported("lb_degree_bins"); }
 int collect_timings(Graph &, int64_t, uint64_t, float *, float *, int64_t *) { return unsupported("collect_timings"); }
 int mlp_train_step(float *, float *, float *, int32_t, const float *, const float *, int64_t, float,
                    float *, float *) { return unsupported("mlp_train_step"); }

@@ -116,7 +116,9 @@ typedef struct {
     int32_t ratio_x16;     /* fixed heuristic: bsearch when short*log2(long)*16 <    */
                            /*   long*ratio_x16                                        */
     int32_t persistent;    /* 1: persistent job-queue launch                          */
-    int32_t reserved[9];
+    int32_t flush_items;   /* min items per owner slot in one job (bounds the per-job */
+                           /*   flush of third-vertex counters)                      */
+    int32_t reserved[8];
 } tc_schedule;
 
 /* Fill *s with the defaults (autotuned table if one was loaded). */
@@ -221,8 +223,17 @@ int tc_gen_rmat_pairs(int32_t scale, int64_t m, double a, double b, double c, ui
                       int32_t *u_dev, int32_t *v_dev, void *hip_stream);
 int tc_gen_er_pairs(int32_t n, int64_t m, uint64_t seed, int32_t *u_dev, int32_t *v_dev,
                     void *hip_stream);
+/* Batched R-MAT: graph i (of G) has n[i] vertices, ef[i]*n[i] pairs drawn on
+ * 2^ceil(log2 n[i]) with seed seeds[i]; pairs with an endpoint >= n[i] are
+ * rejected (written as a self-loop, which canonicalisation drops).  Ids are
+ * global (offset by the sum of the previous n).  Host arrays n, ef, seeds;
+ * device outputs u, v hold sum(ef[i]*n[i]) pairs. */
+int tc_gen_rmat_multi(int32_t G, const int32_t *n, const int32_t *ef, const uint64_t *seeds,
+                      double a, double b, double c, int32_t *u_dev, int32_t *v_dev,
+                      void *hip_stream);
 /* Builds a canonical CSR from m device pairs; *off_dev (int64[nv+1]) and
- * *adj_dev are allocated by the call (free with tc_device_free). */
+ * *adj_dev are allocated by the call (free with tc_device_free).  Pairs with
+ * an endpoint outside [0,nv) are dropped like self-loops. */
 int tc_canon_csr_device(int32_t nv, int64_t m, const int32_t *u_dev, const int32_t *v_dev,
                         int64_t **off_dev, int32_t **adj_dev, int64_t *nnz, void *hip_stream);
 

@@ -120,6 +120,42 @@ def rmat_graph(scale: int, edge_factor: int, seed: int, abc=(0.57, 0.19, 0.19)):
     return canon_csr(n, u, v)
 
 
+def batched_spec(G: int = 4096, seed_root: int = 1000, nmin: int = 1024, nmax: int = 65536,
+                 efmin: int = 4, efmax: int = 32):
+    """Sizes of the batched workload: graph i has n_i = floor(exp(U(ln nmin, ln nmax)))
+    vertices and edge factor ef_i ~ U{efmin..efmax}, both drawn from rng(seed_root+i, .),
+    and is generated with seed seed_root+i."""
+    import math
+    n = np.empty(G, np.int32)
+    ef = np.empty(G, np.int32)
+    seeds = np.empty(G, np.uint64)
+    lo, hi = math.log(nmin), math.log(nmax)
+    for i in range(G):
+        u01 = (rng(seed_root + i, 0) >> 11) / float(1 << 53)
+        n[i] = min(nmax, max(nmin, int(math.floor(math.exp(lo + u01 * (hi - lo))))))
+        ef[i] = efmin + rng(seed_root + i, 1) % (efmax - efmin + 1)
+        seeds[i] = seed_root + i
+    return n, ef, seeds
+
+
+def batched_graph(n, ef, seeds, abc=(0.57, 0.19, 0.19)):
+    """Host build of the concatenated CSR: graph i is R-MAT on 2^ceil(log2 n_i) with
+    pairs having an endpoint >= n_i rejected; ids offset by the previous n."""
+    offs, adjs, voff = [], [], [0]
+    base_e = 0
+    for ni, ei, si in zip(n, ef, seeds):
+        sc = max(1, int(np.ceil(np.log2(ni))))
+        u, v = rmat_pairs(sc, int(ei) * int(ni), int(si), abc)
+        o, a = canon_csr(int(ni), u, v)
+        offs.append(o[:-1] + base_e)
+        adjs.append(a + voff[-1])
+        base_e += len(a)
+        voff.append(voff[-1] + int(ni))
+    off = np.concatenate(offs + [np.array([base_e], np.int64)])
+    adj = np.concatenate(adjs).astype(np.int32) if adjs else np.zeros(0, np.int32)
+    return np.array(voff, np.int64), off, adj
+
+
 def csr_from_edges(n: int, edges) -> tuple:
     """Symmetric simple CSR from an undirected edge iterable."""
     e = np.asarray(list(edges), dtype=np.int64).reshape(-1, 2)

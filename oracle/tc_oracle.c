@@ -114,7 +114,8 @@ int64_t tco_canon_csr(int64_t n, int64_t m, const int32_t *u, const int32_t *v,
     uint64_t *t = (uint64_t *)malloc((size_t)(2 * m + 1) * sizeof(uint64_t));
     int64_t nk = 0;
     for (int64_t i = 0; i < m; ++i) {
-        if (u[i] == v[i]) continue;
+        /* self-loops and pairs with an endpoint outside [0,n) are dropped */
+        if (u[i] == v[i] || u[i] < 0 || v[i] < 0 || u[i] >= n || v[i] >= n) continue;
         k[nk++] = ((uint64_t)(uint32_t)u[i] << 32) | (uint32_t)v[i];
         k[nk++] = ((uint64_t)(uint32_t)v[i] << 32) | (uint32_t)u[i];
     }
