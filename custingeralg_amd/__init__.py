@@ -41,7 +41,8 @@ EXPORTS = [
     "tc_memcpy", "tc_memset", "tc_device_synchronize", "tc_device_name",
     "tc_graph_create_csr", "tc_graph_destroy", "tc_graph_set_stream", "tc_graph_get_info",
     "tc_graph_to_csr", "tc_graph_degrees", "tc_schedule_default", "tc_static_count",
-    "tc_csr_count_device", "tc_batch_apply", "tc_batch_update_only", "tc_stream_delta_passes",
+    "tc_csr_count_device", "tc_batch_apply", "tc_batch_update_only", "tc_batch_prepare",
+    "tc_stream_delta_passes",
     "tc_batched_count", "tc_lb_partition", "tc_lb_degree_bins", "tc_gen_rmat_pairs",
     "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune",
@@ -126,6 +127,7 @@ def lib():
                                      C.POINTER(BatchStats)]
         L.tc_batch_update_only.argtypes = [vp, vp, vp, vp, i64, C.POINTER(BatchStats)]
         L.tc_stream_delta_passes.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32]
+        L.tc_batch_prepare.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.tc_batched_count.argtypes = [i32, vp, vp, vp, i64, i32, vp, vp, C.POINTER(Schedule),
                                        C.POINTER(Stats), vp]
         L.tc_lb_partition.argtypes = [vp, vp, i64, vp, C.POINTER(i64)]
@@ -325,6 +327,22 @@ def batch_apply(g: cuStinger, u, v, is_delete=None, delta: DeviceArray | None = 
         return int(dg.value), st.as_dict()
     check(lib().tc_batch_update_only(g.h, du.ptr, dv.ptr, dd.ptr if dd else None, B, C.byref(st)))
     return 0, st.as_dict()
+
+
+def batch_prepare(g: cuStinger, src, dst):
+    """BatchUpdate::sortDeviceBUD plus duplicate flags against the current
+    store (tests/cct_upd.cpp:395, cct_upd.cu:252): returns the sorted directed
+    entries, dup flags and per-vertex batch offsets as host arrays."""
+    src = np.ascontiguousarray(src, np.int32)
+    dst = np.ascontiguousarray(dst, np.int32)
+    nb = len(src)
+    nv = g.nv
+    ds, dd = DeviceArray.from_host(src), DeviceArray.from_host(dst)
+    s_src, s_dst, dup = (DeviceArray(max(nb, 1), np.int32) for _ in range(3))
+    boff = DeviceArray(nv + 1, np.int64)
+    check(lib().tc_batch_prepare(g.h, nb, ds.ptr, dd.ptr, s_src.ptr, s_dst.ptr, dup.ptr,
+                                 boff.ptr))
+    return (s_src.to_host()[:nb], s_dst.to_host()[:nb], dup.to_host()[:nb], boff.to_host())
 
 
 def lb_partition(g: cuStinger, verts) -> np.ndarray:

@@ -18,6 +18,8 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
                 bool count);
 int stream_delta_passes(Graph &g, int64_t nb, const int32_t *s_src, const int32_t *s_dst,
                         const int32_t *dup, const int64_t *boff, uint32_t *out, int32_t mask);
+int batch_prepare(Graph &g, int64_t nb, const int32_t *src, const int32_t *dst, int32_t *s_src,
+                  int32_t *s_dst, int32_t *dup, int64_t *bo);
 int batched_count(int32_t G, const int64_t *voff, const int64_t *off, const int32_t *adj,
                   int64_t nnz, int32_t nv_total, uint64_t *per_graph, uint32_t *per_vertex,
                   const tc_schedule &sched, tc_stats *stats, hipStream_t s);
@@ -86,7 +88,7 @@ int tc_device_alloc(void **ptr, uint64_t bytes)
 {
     TC_CHECK_ARG(ptr, "tc_device_alloc: null ptr");
     if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) {
-        hipGetLastError();
+        (void)hipGetLastError();
         *ptr = nullptr;
         tc::set_last_error(TC_ERR_OOM, "hipMalloc failed");
         return TC_ERR_OOM;
@@ -284,6 +286,16 @@ int tc_batch_update_only(tc_graph *g, const int32_t *u, const int32_t *v,
         TC_CHECK_ARG(g && B >= 0 && (B == 0 || (u && v)), "tc_batch_update_only: bad arguments");
         tc_schedule s = tc::default_schedule();
         return tc::batch_apply(*g->g, u, v, is_delete, B, nullptr, nullptr, s, stats, false);
+    })
+}
+
+int tc_batch_prepare(tc_graph *g, int64_t nb, const int32_t *src, const int32_t *dst,
+                     int32_t *s_src, int32_t *s_dst, int32_t *dup, int64_t *boff)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(g && nb >= 0 && (nb == 0 || (src && dst && s_src && s_dst && dup)) && boff,
+                     "tc_batch_prepare: bad arguments");
+        return tc::batch_prepare(*g->g, nb, src, dst, s_src, s_dst, dup, boff);
     })
 }
 

@@ -298,8 +298,8 @@ int canon_csr_device(int32_t nv, int64_t m, const int32_t *u, const int32_t *v, 
     return TC_OK;
 fail:
     cleanup();
-    if (off) hipFree(off);
-    if (adj) hipFree(adj);
+    if (off) (void)hipFree(off);
+    if (adj) (void)hipFree(adj);
     return st;
 }
 

@@ -22,14 +22,14 @@ int DevBuf::ensure(size_t need)
     if (need == 0) need = 16;
     if (bytes >= need) return TC_OK;
     if (p) {
-        hipFree(p);
+        (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
     hipError_t e = hipMalloc(&p, need);
     if (e != hipSuccess) {
         p = nullptr;
-        hipGetLastError();
+        (void)hipGetLastError();
         set_last_error(TC_ERR_OOM, "hipMalloc of " + std::to_string(need) + " bytes failed");
         return TC_ERR_OOM;
     }
@@ -39,7 +39,7 @@ int DevBuf::ensure(size_t need)
 
 void DevBuf::release()
 {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
 }
@@ -272,8 +272,11 @@ int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t 
         }
         if (location == TC_DEVICE) {
             int64_t ends[2];
-            hipMemcpy(&ends[0], doff, sizeof(int64_t), hipMemcpyDeviceToHost);
-            hipMemcpy(&ends[1], doff + nv, sizeof(int64_t), hipMemcpyDeviceToHost);
+            if (hipMemcpy(&ends[0], doff, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(&ends[1], doff + nv, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess) {
+                set_last_error(TC_ERR_HIP, "read-back of the offsets failed");
+                return fail(TC_ERR_HIP);
+            }
             if (ends[0] != 0 || ends[1] != nnz) {
                 set_last_error(TC_ERR_INVALID, "tc_graph_create_csr: off[0] must be 0 and off[nv] == nnz");
                 return fail(TC_ERR_INVALID);
@@ -291,8 +294,13 @@ int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t 
         if ((st = scan.ensure((nvb + 1) * sizeof(int64_t))) != TC_OK) return fail(st);
         if ((st = scan_counts_i32(g->st.cap.as<int32_t>(), scan.as<int64_t>(), nv, tmp, s)) != TC_OK)
             return fail(st);
-        hipMemcpy(&pool_n, scan.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost);
-        hipMemcpy(g->st.boff.p, scan.p, (size_t)nv * sizeof(int64_t), hipMemcpyDeviceToDevice);
+        if (hipMemcpy(&pool_n, scan.as<int64_t>() + nv, sizeof(int64_t), hipMemcpyDeviceToHost) !=
+                hipSuccess ||
+            hipMemcpy(g->st.boff.p, scan.p, (size_t)nv * sizeof(int64_t), hipMemcpyDeviceToDevice) !=
+                hipSuccess) {
+            set_last_error(TC_ERR_HIP, "copy of the block offsets failed");
+            return fail(TC_ERR_HIP);
+        }
         tmp.release();
         scan.release();
     }
@@ -313,12 +321,19 @@ int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t 
             DevBuf stage;
             if (location == TC_HOST) {
                 if ((st = stage.ensure((size_t)nnz * sizeof(int32_t))) != TC_OK) return fail(st);
-                hipMemcpy(stage.p, adj, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice);
+                if (hipMemcpy(stage.p, adj, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice) !=
+                    hipSuccess) {
+                    set_last_error(TC_ERR_HIP, "copy of adjacency failed");
+                    return fail(TC_ERR_HIP);
+                }
                 dadj = stage.as<int32_t>();
             }
             hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(nv, 4, 65536)), dim3(256), 0, s, doff, dadj,
                                g->st.boff.as<int64_t>(), nv, g->st.pool.as<int32_t>());
-            hipDeviceSynchronize();
+            if (hipDeviceSynchronize() != hipSuccess) {
+                set_last_error(TC_ERR_HIP, "row copy failed");
+                return fail(TC_ERR_HIP);
+            }
             stage.release();
         }
     }
@@ -413,7 +428,7 @@ void graph_destroy(Graph *g)
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : g->sb) b.release();
     if (g->events_ready)
-        for (auto &e : g->ev) hipEventDestroy(e);
+        for (auto &e : g->ev) (void)hipEventDestroy(e);
     delete g;
 }
 

@@ -129,6 +129,8 @@ struct Graph {
     int ensure_events();
 };
 
+// Builds coff, crow and vinfo (and padj for the oriented layout) for the store.
+int prepare_views(Graph &g, bool oriented);
 // Fills vinfo with the store's rows as given (direct layout, dm=0).
 int build_direct_vinfo(Graph &g);
 // Validation of sortedness/range, self-loop count, optional symmetry check.

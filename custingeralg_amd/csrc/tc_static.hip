@@ -849,6 +849,58 @@ int build_direct_vinfo(Graph &g)
     return TC_OK;
 }
 
+int prepare_views(Graph &g, bool oriented)
+{
+    const int32_t nv = g.st.nv;
+    const int64_t nnz = g.st.nnz;
+    hipStream_t s = g.stream;
+    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
+    unsigned long long *acc = g.accum.as<unsigned long long>();
+    // compact offsets of the store rows
+    TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
+    int64_t *coff = g.coff.as<int64_t>();
+    TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), coff, nv, g.tmp, s));
+
+    TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
+    VInfo *vi = g.vinfo.as<VInfo>();
+    const int32_t *adj = nullptr;
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
+    int32_t *crow = g.crow.as<int32_t>();
+    hipLaunchKernelGGL(k_chunk_rows, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff, nv, crow);
+    const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
+    if (oriented) {
+        TC_TRY(g.padj.ensure((size_t)nnz * sizeof(int32_t)));
+        // scratch: mask (u64 x nchunks), cnt (i32 x nchunks), cscan (i64 x nchunks+1)
+        size_t need = (size_t)nchunks * 8 + (size_t)nchunks * 4 + (size_t)(nchunks + 1) * 8 + 64;
+        TC_TRY(g.tmp2.ensure(need));
+        uint64_t *mask = g.tmp2.as<uint64_t>();
+        int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
+        int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
+        int32_t *maxplus = reinterpret_cast<int32_t *>(acc + 15);
+        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, coff, crow,
+                           g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
+                           nv, nnz, mask, cnt);
+        TC_HIP(hipGetLastError());
+        TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
+        hipLaunchKernelGGL(k_orient_vinfo, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff,
+                           g.st.len.as<int32_t>(), cscan, mask, nv, vi, maxplus);
+        TC_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_orient_write, dim3(wblocks), dim3(256), 0, s, coff, crow,
+                           g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, vi, nv,
+                           nnz, g.padj.as<int32_t>());
+        TC_HIP(hipGetLastError());
+        adj = g.padj.as<int32_t>();
+    } else {
+        TC_TRY(build_direct_vinfo(g));
+        adj = g.st.pool.as<int32_t>();
+    }
+
+    (void)adj;
+    (void)acc;
+    return TC_OK;
+}
+
 extern const uint8_t *learned_table_device(int32_t *dim);
 
 int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &sched,
@@ -899,46 +951,14 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         return TC_OK;
     }
 
-    // compact offsets of the store rows
-    TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
+    // compact offsets, chunk rows and the (degree-partitioned) vertex views
+    TC_TRY(prepare_views(g, oriented));
     int64_t *coff = g.coff.as<int64_t>();
-    TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), coff, nv, g.tmp, s));
-
-    TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
-    VInfo *vi = g.vinfo.as<VInfo>();
-    const int32_t *adj = nullptr;
-    const int64_t nchunks = ceil_div(nnz, kWave);
-    TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
     int32_t *crow = g.crow.as<int32_t>();
-    hipLaunchKernelGGL(k_chunk_rows, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff, nv, crow);
+    VInfo *vi = g.vinfo.as<VInfo>();
+    const int32_t *adj = oriented ? g.padj.as<int32_t>() : g.st.pool.as<int32_t>();
+    const int64_t nchunks = ceil_div(nnz, kWave);
     const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
-    int32_t maxplus_h = 0;
-    if (oriented) {
-        TC_TRY(g.padj.ensure((size_t)nnz * sizeof(int32_t)));
-        // scratch: mask (u64 x nchunks), cnt (i32 x nchunks), cscan (i64 x nchunks+1)
-        size_t need = (size_t)nchunks * 8 + (size_t)nchunks * 4 + (size_t)(nchunks + 1) * 8 + 64;
-        TC_TRY(g.tmp2.ensure(need));
-        uint64_t *mask = g.tmp2.as<uint64_t>();
-        int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
-        int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
-        int32_t *maxplus = reinterpret_cast<int32_t *>(acc + 15);
-        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, coff, crow,
-                           g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
-                           nv, nnz, mask, cnt);
-        TC_HIP(hipGetLastError());
-        TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
-        hipLaunchKernelGGL(k_orient_vinfo, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff,
-                           g.st.len.as<int32_t>(), cscan, mask, nv, vi, maxplus);
-        TC_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_orient_write, dim3(wblocks), dim3(256), 0, s, coff, crow,
-                           g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, vi, nv,
-                           nnz, g.padj.as<int32_t>());
-        TC_HIP(hipGetLastError());
-        adj = g.padj.as<int32_t>();
-    } else {
-        TC_TRY(build_direct_vinfo(g));
-        adj = g.st.pool.as<int32_t>();
-    }
 
     // ---- planning: work per owner, jobs per bin
     PlanParams P;
@@ -1025,14 +1045,14 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     if (global) *global = hacc[0];
     if (stats) {
         float ms = 0;
-        hipEventElapsedTime(&ms, g.ev[0], g.ev[6]);
+        (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[6]);
         st.ms_total = ms;
-        hipEventElapsedTime(&ms, g.ev[0], g.ev[2]);
+        (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[2]);
         st.ms_prepare = ms;
-        hipEventElapsedTime(&ms, g.ev[3], g.ev[6]);
+        (void)hipEventElapsedTime(&ms, g.ev[3], g.ev[6]);
         st.ms_count = ms;
         for (int b = 0; b < 3; ++b) {
-            hipEventElapsedTime(&ms, g.ev[3 + b], g.ev[4 + b]);
+            (void)hipEventElapsedTime(&ms, g.ev[3 + b], g.ev[4 + b]);
             st.ms_bin[b] = ms;
             st.jobs_bin[b] = jt[b];
             st.tasks_bin[b] = (int64_t)hacc[4 + 2 * b];
@@ -1042,7 +1062,6 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         st.triangles = oriented ? hacc[0] / 6 : 0;
         *stats = st;
     }
-    (void)maxplus_h;
     return TC_OK;
 }
 

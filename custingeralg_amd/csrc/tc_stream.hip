@@ -716,7 +716,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_HIP(hipMemcpyAsync(&hq, acc + 4, sizeof(hq), hipMemcpyDeviceToHost, s));
         TC_HIP(hipStreamSynchronize(s));
         if (count) {
-            hipEventElapsedTime(&ms, g.ev[9], g.ev[10]);
+            (void)hipEventElapsedTime(&ms, g.ev[9], g.ev[10]);
             ms_delta += ms;
         }
         if (hq > 0) {
@@ -813,14 +813,14 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
             TC_HIP(hipGetLastError());
             TC_HIP(hipStreamSynchronize(s));
             if (count) {
-                hipEventElapsedTime(&ms, g.ev[11], g.ev[12]);
+                (void)hipEventElapsedTime(&ms, g.ev[11], g.ev[12]);
                 ms_delta += ms;
             }
         }
     }
     TC_HIP(hipEventRecord(g.ev[13], s));
     TC_HIP(hipEventSynchronize(g.ev[13]));
-    hipEventElapsedTime(&ms, g.ev[8], g.ev[13]);
+    (void)hipEventElapsedTime(&ms, g.ev[8], g.ev[13]);
     bs.ms_delta = ms_delta;
     bs.ms_update = ms - ms_delta;
     bs.missing_deletes = nD - bs.deleted;
@@ -843,6 +843,95 @@ int stream_delta_passes(Graph &g, int64_t nb, const int32_t *s_src, const int32_
                        g.st.len.as<int32_t>(), out, mask);
     TC_HIP(hipGetLastError());
     TC_HIP(hipStreamSynchronize(g.stream));
+    return TC_OK;
+}
+
+}  // namespace tc
+
+// ---------------------------------------------------------------------------
+// Batch preparation for the reference-shaped streaming flow
+// (BatchUpdate::sortDeviceBUD, tests/cct_upd.cpp:395): directed entries sorted
+// by (src, dst), dup[i] = 1 for in-batch repeats (all but the first) and for
+// entries already in the store, boff[v] = first sorted entry with src >= v.
+namespace tc {
+namespace {
+
+__global__ void k_dir_keys(const int32_t *__restrict__ src, const int32_t *__restrict__ dst,
+                           int64_t nb, int bv, uint64_t *__restrict__ keys)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = ((uint64_t)(uint32_t)src[i] << bv) | (uint32_t)dst[i];
+}
+
+__global__ void k_prep_flags(const uint64_t *__restrict__ keys, int64_t nb, int bv,
+                             const int32_t *__restrict__ pool, const int64_t *__restrict__ boff,
+                             const int32_t *__restrict__ len, int32_t *__restrict__ s_src,
+                             int32_t *__restrict__ s_dst, int32_t *__restrict__ dup)
+{
+    const uint64_t m = (1ull << bv) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const int32_t a = (int32_t)(k >> bv), b = (int32_t)(k & m);
+        s_src[i] = a;
+        s_dst[i] = b;
+        const bool rep = i > 0 && keys[i - 1] == k;
+        dup[i] = (rep || find_pos(pool, boff[a], len[a], b) >= 0) ? 1 : 0;
+    }
+}
+
+__global__ void k_prep_offsets(const int32_t *__restrict__ s_src, int64_t nb, int32_t nv,
+                               int64_t *__restrict__ bo)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = nb;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (s_src[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        bo[v] = lo;
+    }
+}
+
+}  // namespace
+
+int batch_prepare(Graph &g, int64_t nb, const int32_t *src, const int32_t *dst, int32_t *s_src,
+                  int32_t *s_dst, int32_t *dup, int64_t *bo)
+{
+    const int32_t nv = g.st.nv;
+    hipStream_t s = g.stream;
+    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
+    unsigned long long *acc = g.accum.as<unsigned long long>();
+    hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc, 16);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_batch_check, dim3(grid_for(nb, 256, 4096)), dim3(256), 0, s, src, dst, nb,
+                           nv, acc);
+        unsigned long long hb[2];
+        TC_HIP(hipMemcpyAsync(hb, acc, sizeof(hb), hipMemcpyDeviceToHost, s));
+        TC_HIP(hipStreamSynchronize(s));
+        if (hb[0]) {
+            set_last_error(TC_ERR_INVALID, "batch has out-of-range ids");
+            return TC_ERR_INVALID;
+        }
+    }
+    int bv = 1;
+    while ((int64_t(1) << bv) < nv) ++bv;
+    if (nb > 0) {
+        uint64_t *keys = buf<uint64_t>(g, 0, nb);
+        TC_NEED(keys);
+        hipLaunchKernelGGL(k_dir_keys, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, s, src, dst, nb, bv,
+                           keys);
+        TC_TRY(radix_sort_u64(keys, nullptr, nb, 2 * bv, g.sb[1], g.sb[2], g.tmp, s));
+        hipLaunchKernelGGL(k_prep_flags, dim3(grid_for(nb, 256, 65536)), dim3(256), 0, s, keys, nb, bv,
+                           g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                           s_src, s_dst, dup);
+    }
+    hipLaunchKernelGGL(k_prep_offsets, dim3(grid_for((int64_t)nv + 1, 256, 65536)), dim3(256), 0, s,
+                       s_src, nb, nv, bo);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipStreamSynchronize(s));
     return TC_OK;
 }
 

@@ -1,0 +1,157 @@
+"""Learned intersection-strategy selector and schedule autotuning.
+
+The cost model is a 4 -> 32 -> 32 -> 3 MLP over per-task features
+(log2 |owner tile| / 16, log2 |task list| / 16, log2 ratio / 16, bin / 2)
+predicting log2(cycles) of the merge, binary-search and hash strategies.
+Timings come from the method itself (``collect_timings``), training runs on
+the device (``tc_mlp_train_step``: FP32 forward/backward + Adam), and
+``load`` rounds the weights to BF16 and fills the strategy table that
+``POLICY_LEARNED`` reads.  Counts never depend on the strategy.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from . import (MLP_PARAMS, POLICY_FIXED, POLICY_LEARNED, DeviceArray, Schedule, check,
+               cuStinger, lib, static_count)
+
+
+def collect_timings(g: cuStinger, max_samples: int = 4096, seed: int = 1):
+    """Per-task features (n, 4) and cycles per strategy (n, 3: merge, bsearch, hash)."""
+    feat = np.zeros((max_samples, 4), np.float32)
+    cost = np.zeros((max_samples, 3), np.float32)
+    n = C.c_int64(0)
+    check(lib().tc_collect_timings(g.h, max_samples, seed, feat.ctypes.data, cost.ctypes.data,
+                                   C.byref(n)))
+    return feat[:n.value].copy(), cost[:n.value].copy()
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-initialised weights, zero biases, in the device layout."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for shape, w in (((32, 4), True), ((32,), False), ((32, 32), True), ((32,), False),
+                     ((3, 32), True), ((3,), False)):
+        parts.append(rng.normal(0, np.sqrt(2.0 / shape[1]), shape).ravel() if w
+                     else np.zeros(shape))
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == MLP_PARAMS
+    return p
+
+
+class CostModel:
+    """Device-trained cost model."""
+
+    def __init__(self, params: np.ndarray | None = None, seed: int = 0):
+        p = init_params(seed) if params is None else np.ascontiguousarray(params, np.float32)
+        self.params = DeviceArray.from_host(p)
+        self.m = DeviceArray(MLP_PARAMS, np.float32).zero()
+        self.v = DeviceArray(MLP_PARAMS, np.float32).zero()
+        self.step = 0
+
+    def host_params(self) -> np.ndarray:
+        return self.params.to_host()
+
+    def train_step(self, feat_dev: DeviceArray, target_dev: DeviceArray, n: int, lr: float,
+                   adam: bool = True, grads: DeviceArray | None = None) -> float:
+        self.step += 1
+        loss = C.c_float(0)
+        check(lib().tc_mlp_train_step(self.params.ptr, self.m.ptr if adam else None,
+                                      self.v.ptr if adam else None, self.step, feat_dev.ptr,
+                                      target_dev.ptr, n, lr, grads.ptr if grads else None,
+                                      C.byref(loss)))
+        return float(loss.value)
+
+    def fit(self, feat: np.ndarray, target: np.ndarray, steps: int = 300, lr: float = 0.01):
+        fd = DeviceArray.from_host(np.ascontiguousarray(feat, np.float32))
+        td = DeviceArray.from_host(np.ascontiguousarray(target, np.float32))
+        losses = [self.train_step(fd, td, len(feat), lr) for _ in range(steps)]
+        return losses
+
+    def predict(self, feat: np.ndarray) -> np.ndarray:
+        fd = DeviceArray.from_host(np.ascontiguousarray(feat, np.float32))
+        out = DeviceArray(len(feat) * 3, np.float32)
+        check(lib().tc_mlp_forward(self.params.ptr, fd.ptr, len(feat), out.ptr))
+        return out.to_host().reshape(-1, 3)
+
+    def load(self) -> np.ndarray:
+        """Make this model the learned policy; returns the strategy table."""
+        p = self.host_params()
+        check(lib().tc_model_load(p.ctypes.data))
+        return model_table()
+
+    def save(self, path: str) -> None:
+        with open(path, "w") as f:
+            json.dump({"params": self.host_params().tolist(), "step": self.step}, f)
+
+    @classmethod
+    def from_file(cls, path: str) -> "CostModel":
+        with open(path) as f:
+            d = json.load(f)
+        return cls(np.array(d["params"], np.float32))
+
+
+def targets_from_cycles(cost: np.ndarray) -> np.ndarray:
+    """Training target: log2(cycles) / 16 per strategy."""
+    return (np.log2(np.maximum(cost, 1.0)) / 16.0).astype(np.float32)
+
+
+def model_table() -> np.ndarray:
+    dim = C.c_int32(0)
+    check(lib().tc_model_table(None, C.byref(dim)))
+    t = np.zeros(dim.value * dim.value, np.uint8)
+    check(lib().tc_model_table(t.ctypes.data, C.byref(dim)))
+    return t.reshape(dim.value, dim.value)
+
+
+def unload() -> None:
+    check(lib().tc_model_load(None))
+
+
+def train_selector(graphs, samples_per_graph: int = 4096, steps: int = 400, lr: float = 0.01,
+                   seed: int = 0):
+    """Collect timings on each graph, train a model on the device and load it."""
+    feats, costs = [], []
+    for i, g in enumerate(graphs):
+        f, c = collect_timings(g, samples_per_graph, seed + 17 * i + 1)
+        feats.append(f)
+        costs.append(c)
+    feat = np.concatenate(feats)
+    cost = np.concatenate(costs)
+    model = CostModel(seed=seed)
+    losses = model.fit(feat, targets_from_cycles(cost), steps, lr)
+    table = model.load()
+    return model, table, losses, (feat, cost)
+
+
+def autotune(g: cuStinger):
+    """Sweep bin thresholds and job grains; the best becomes the default schedule."""
+    s = Schedule()
+    ms = C.c_double(0)
+    check(lib().tc_autotune(g.h, C.byref(s), C.byref(ms)))
+    return s, float(ms.value)
+
+
+def compare_policies(g: cuStinger, reps: int = 3):
+    """Best-of-reps time of the fixed heuristic and the learned table on one graph."""
+    from . import default_schedule
+    out = {}
+    ref = None
+    for name, pol in (("fixed", POLICY_FIXED), ("learned", POLICY_LEARNED)):
+        best = None
+        for _ in range(reps):
+            _, glob, st = static_count(g, default_schedule(policy=pol), per_vertex=False)
+            if ref is None:
+                ref = glob
+            assert glob == ref, "policies disagree on the count"
+            best = st if best is None or st["ms_total"] < best["ms_total"] else best
+        out[name] = best
+    return out
+
+
+def default_model_path() -> str:
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.json")

@@ -184,6 +184,15 @@ int tc_batch_apply(tc_graph *g, const int32_t *u, const int32_t *v, const uint8_
 int tc_batch_update_only(tc_graph *g, const int32_t *u, const int32_t *v,
                          const uint8_t *is_delete, int64_t B, tc_batch_stats *stats);
 
+/* BatchUpdate::sortDeviceBUD (tests/cct_upd.cpp:395) with the duplicate
+ * flags the delta kernels consume (cct_upd.cu:252,308,364): the nb directed
+ * device entries (src, dst) are sorted by (src, dst) into s_src/s_dst;
+ * dup[i] = 1 for in-batch repeats (all but the first) and for entries already
+ * in the store; boff (device int64[nv+1]) holds each vertex's first sorted
+ * entry.  Call before inserting the batch. */
+int tc_batch_prepare(tc_graph *g, int64_t nb, const int32_t *src, const int32_t *dst,
+                     int32_t *s_src, int32_t *s_dst, int32_t *dup, int64_t *boff);
+
 /* Reference-faithful delta kernels on a prepared batch, for parity of the
  * three passes (cct_upd.cu:231-410): s_src/s_dst sorted directed entries,
  * dup flags (device int32[nb]), boff int64[nv+1] batch offsets; the graph

@@ -192,3 +192,31 @@ def test_rmat16_twenty_batches_recount():
         ni = int(0.8 * b)
         batches.append((ni, b - ni, 14 if i % 2 else None))
     _run_batches(off, adj, batches, 0, 11, check_every=5)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_batch_prepare_matches_oracle(seed):
+    """tc_batch_prepare (sortDeviceBUD + duplicate flags) against the oracle on a
+    batch holding in-batch repeats and entries already in the store."""
+    off, adj = O.rmat_graph(9, 8, 6 + seed)
+    n = len(off) - 1
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, 500).astype(np.int32)
+    v = rng.integers(0, n, 500).astype(np.int32)
+    k = u != v
+    u, v = u[k], v[k]
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(off))
+    u = np.concatenate([u, u[:50], rows[::37]])
+    v = np.concatenate([v, v[:50], adj[::37]])
+    src = np.concatenate([u, v]).astype(np.int32)
+    dst = np.concatenate([v, u]).astype(np.int32)
+    g = T.cuStinger(off, adj)
+    got = T.batch_prepare(g, src, dst)
+    want = O.batch_prepare(n, src, dst, off, adj)
+    for a, b, name in zip(got, want, ("s_src", "s_dst", "dup", "boff")):
+        assert np.array_equal(a, b), name
+    assert got[2].sum() >= 50  # the planted repeats are flagged
+    # empty batch: all offsets zero
+    e = T.batch_prepare(g, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    assert len(e[0]) == 0 and not e[3].any()
+    g.close()
