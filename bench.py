@@ -1,23 +1,41 @@
-"""Static triangle-count benchmark on one MI355X.
+"""Triangle-count benchmarks on one MI355X.
 
-Workload: R-MAT scale 24, edge factor 32, (a,b,c,d)=(0.57,0.19,0.19,0.05),
-seed 4, symmetrised / deduplicated / loop-free, generated on the device with
-the project's counter-based generator (synthetic data; nothing is read from
-disk).  A "step" is one full static count of that graph -- orientation,
-job planning and the intersection kernels -- producing the uint32
-per-vertex counts and the uint64 global count.  value = nnz / step time.
+Default workload (static): R-MAT scale 24, edge factor 32,
+(a,b,c,d)=(0.57,0.19,0.19,0.05), seed 4, symmetrised / deduplicated /
+loop-free, generated on the device with the project's counter-based
+generator (synthetic data; nothing is read from disk).  A "step" is one full
+static count of that graph -- orientation, job planning and the intersection
+kernels -- producing the uint32 per-vertex counts (kept in HBM) and the uint64
+global count.  Three legs run on the same graph, each K timed steps:
+
+  fixed    built-in schedule, fixed degree-ratio strategy heuristic;
+  tuned    autotuned bin thresholds / job grains, fixed heuristic;
+  learned  autotuned schedule with the learned strategy table (MLP trained on
+           the device from intersection timings of R-MAT scale 18, 20, 22).
+
+value = nnz / step time of the learned leg; all three must give the same
+global count.  The roofline object describes the dominant kernel (the count
+kernel of the bin with the largest event time) in that leg.
+
+Other workloads (``--workload``): ``stream`` (R-MAT scale 22 ef 16 seed 3 base,
+20 update batches of 10^3..10^6 pairs, updates/s, running total checked against
+a recount after every batch), ``batched`` (4096 R-MAT graphs of 1k-64k
+vertices counted in one call, edges/s) and ``copy`` (a plain device copy of
+known size, the calibration kernel for the HBM traffic counters).
 
 The CPU baseline times the oracle's restatement of the reference CPU path
 (hostCountTriangles' per-entry merge, tests/cct.cpp:32-89) on a uniform
 sample of the same graph's entries, on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S] [--ef E]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload static|stream|batched|copy]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,6 +46,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TC edges/sec + HBM GB/s at RMAT-24; learned vs fixed schedule, exact counts"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_count_rmat24_traffic.md")
 
 
 def log(msg: str) -> None:
@@ -59,11 +78,245 @@ def cpu_baseline(off, adj, budget_s: float, seed: int = 7) -> dict:
                       f"{t_all:.1f} s on {threads} threads"}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant count kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def timed_steps(T, step, steps, warmup):
+    """Run warmup + steps calls of step(); returns (ms per step, [stats])."""
+    for _ in range(warmup):
+        step()
+    T.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        stats.append(step())
+    T.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, stats
+
+
+def bench_static(T, args) -> dict:
+    from custingeralg_amd import learned as Lm
+
+    t0 = time.time()
+    g = T.rmat_graph_device(args.scale, args.ef, args.seed)
+    nnz = g.info()["nnz"]
+    log(f"[bench] R-MAT scale {args.scale} ef {args.ef} seed {args.seed}: nnz={nnz} "
+        f"max_degree={g.info()['max_degree']} built in {time.time() - t0:.1f}s")
+    out = T.DeviceArray(g.nv, np.uint32)
+    ref = {}
+
+    def leg(name, policy):
+        sched = T.default_schedule(policy=policy)
+
+        def step():
+            _, glob, st = T.static_count(g, sched, per_vertex=True, out=out)
+            if ref.setdefault("glob", glob) != glob:
+                raise RuntimeError(f"{name}: global count {glob} != {ref['glob']}")
+            return st
+
+        ms, stats = timed_steps(T, step, args.steps, args.warmup)
+        best = min(stats, key=lambda s: s["ms_total"])
+        log(f"[bench] {name}: {ms:.1f} ms/step (events best {best['ms_total']:.1f} ms, "
+            f"bins {[round(x, 1) for x in best['ms_bin']]})")
+        return {"ms_per_step": ms, "stats": best}
+
+    legs = {"fixed": leg("fixed", T.POLICY_FIXED)}
+
+    # learned selector: timings from smaller R-MAT graphs, trained on the device
+    t0 = time.time()
+    train = [T.rmat_graph_device(s, 16, 100 + s) for s in (18, 20, 22)]
+    _, table, losses, (feat, _) = Lm.train_selector(train, samples_per_graph=4096, steps=400)
+    tuned, tune_ms = Lm.autotune(train[-1])
+    for tg in train:
+        tg.close()
+    picks = np.bincount(table.ravel(), minlength=3).tolist()
+    log(f"[bench] selector: {len(feat)} timed tasks, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
+        f"table picks merge/bsearch/hash {picks}; autotune small={tuned.small_max} "
+        f"mid={tuned.mid_max} flush={tuned.flush_items} job={tuned.job_items} "
+        f"({time.time() - t0:.1f}s)")
+    legs["tuned"] = leg("tuned", T.POLICY_FIXED)
+    legs["learned"] = leg("learned", T.POLICY_LEARNED)
+    Lm.unload()
+
+    main = legs["learned"]
+    st = main["stats"]
+    b = int(np.argmax(st["ms_bin"]))
+    ms_b = st["ms_bin"][b]
+    achieved = st["bytes_alg_bin"][b] / (ms_b * 1e-3) / 1e9 if ms_b > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        off, adj = g.to_csr()
+        cpu = cpu_baseline(off, adj, args.cpu_budget)
+        del off, adj
+    out.free()
+    g.close()
+    glob = ref["glob"]
+    return {
+        "metric": METRIC, "value": nnz / (main["ms_per_step"] * 1e-3), "unit": "edges/s",
+        "ms_per_step": main["ms_per_step"], "higher_is_better": True, "dtype": "int32",
+        "config": {"workload": f"static TC, R-MAT scale {args.scale} ef {args.ef} "
+                               f"(0.57,0.19,0.19,0.05) seed {args.seed}",
+                   "nnz": nnz, "triangles": glob // 6, "global_count": glob,
+                   "schedule": "autotuned + learned strategy table",
+                   "legs_ms_per_step": {k: v["ms_per_step"] for k, v in legs.items()},
+                   "learned_vs_fixed": legs["fixed"]["ms_per_step"] / main["ms_per_step"],
+                   "tuned": {"small_max": tuned.small_max, "mid_max": tuned.mid_max,
+                             "flush_items": tuned.flush_items, "job_items": tuned.job_items},
+                   "table_picks": picks},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": f"k_count bin {b}", "kernel_ms": ms_b,
+                     "alg_bytes": st["bytes_alg_bin"][b]},
+        "cpu_baseline": cpu,
+        "bins": {"ms": st["ms_bin"], "tasks": st["tasks_bin"], "jobs": st["jobs_bin"],
+                 "alg_bytes": st["bytes_alg_bin"], "strategy_tasks": st["strategy_tasks"],
+                 "ms_prepare": st["ms_prepare"], "ms_count": st["ms_count"]},
+    }
+
+
+def _stream_batch(T, g, nv, B, scale, seed, rng):
+    """80% inserts (half uniform, half R-MAT), 20% deletions of present edges;
+    pairs canonical (min,max), deduplicated, an insert equal to a delete dropped."""
+    n_ins = int(0.8 * B)
+    n_del = B - n_ins
+    nu = n_ins // 2
+    du, dv = T.DeviceArray(nu, np.int32), T.DeviceArray(nu, np.int32)
+    T.check(T.lib().tc_gen_er_pairs(nv, nu, seed, du.ptr, dv.ptr, None))
+    ru, rv = T.DeviceArray(n_ins - nu, np.int32), T.DeviceArray(n_ins - nu, np.int32)
+    T.check(T.lib().tc_gen_rmat_pairs(scale, n_ins - nu, 0.57, 0.19, 0.19, seed + 1, ru.ptr,
+                                      rv.ptr, None))
+    iu = np.concatenate([du.to_host(), ru.to_host()])
+    iv = np.concatenate([dv.to_host(), rv.to_host()])
+    off, adj = g.to_csr()
+    e = rng.integers(0, len(adj), n_del)
+    xu = (np.searchsorted(off, e, side="right") - 1).astype(np.int32)
+    xv = adj[e]
+    del off, adj
+
+    def canon(a, b):
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        keep = lo != hi
+        return np.unique((lo[keep].astype(np.int64) << 32) | hi[keep].astype(np.int64))
+
+    ki, kd = canon(iu, iv), canon(xu, xv)
+    ki = ki[~np.isin(ki, kd)]
+    keys = np.concatenate([ki, kd])
+    u = (keys >> 32).astype(np.int32)
+    v = (keys & 0xFFFFFFFF).astype(np.int32)
+    dl = np.concatenate([np.zeros(len(ki), np.uint8), np.ones(len(kd), np.uint8)])
+    return u, v, dl
+
+
+def bench_stream(T, args) -> dict:
+    scale, ef, seed = 22, 16, 3
+    g = T.rmat_graph_device(scale, ef, seed)
+    nv = g.nv
+    log(f"[bench] stream base R-MAT {scale} ef {ef}: nnz={g.info()['nnz']}")
+    _, total, _ = T.static_count(g, per_vertex=False)
+    delta = T.DeviceArray(nv, np.uint32).zero()
+    rng = np.random.default_rng(seed)
+    sizes = [10 ** 3] * 5 + [10 ** 4] * 5 + [10 ** 5] * 5 + [10 ** 6] * 5
+    per_size = {}
+    t_all, n_all = 0.0, 0
+    for i, B in enumerate(sizes):
+        u, v, dl = _stream_batch(T, g, nv, B, scale, 5000 + 7 * i, rng)
+        du, dv, dd = (T.DeviceArray.from_host(x) for x in (u, v, dl))
+        st = T.BatchStats()
+        dg = C.c_int64(0)
+        T.synchronize()
+        t0 = time.perf_counter()
+        T.check(T.lib().tc_batch_apply(g.h, du.ptr, dv.ptr, dd.ptr, len(u), delta.ptr,
+                                       C.byref(dg), None, C.byref(st)))
+        T.synchronize()
+        dt = time.perf_counter() - t0
+        total += dg.value
+        _, recount, _ = T.static_count(g, per_vertex=False)
+        if recount != total:
+            raise RuntimeError(f"batch {i}: running total {total} != recount {recount}")
+        s = per_size.setdefault(B, {"pairs": 0, "s": 0.0, "ms_update": 0.0, "ms_delta": 0.0})
+        s["pairs"] += len(u)
+        s["s"] += dt
+        s["ms_update"] += st.ms_update
+        s["ms_delta"] += st.ms_delta
+        t_all += dt
+        n_all += len(u)
+        log(f"[bench] batch {i}: B={len(u)} ({int(dl.sum())} deletions) {dt * 1e3:.2f} ms "
+            f"(update {st.ms_update:.2f} delta {st.ms_delta:.2f}) delta={dg.value} ok")
+    g.close()
+    rates = {str(B): {"updates_per_s": s["pairs"] / s["s"],
+                      "ms_update": s["ms_update"] / 5, "ms_delta": s["ms_delta"] / 5}
+             for B, s in per_size.items()}
+    return {"metric": "streaming TC updates/sec (exact delta, checked by recount)",
+            "value": n_all / t_all, "unit": "updates/s", "ms_per_step": t_all / len(sizes) * 1e3,
+            "higher_is_better": True, "dtype": "int32",
+            "config": {"workload": f"streaming TC, R-MAT scale {scale} ef {ef} seed {seed} base, "
+                                   "20 batches 5x10^3..5x10^6, 80% inserts / 20% deletes",
+                       "per_batch_size": rates, "final_global_count": total},
+            "roofline": None, "cpu_baseline": None}
+
+
+def bench_batched(T, args) -> dict:
+    n, ef, seeds = T.batched_spec()
+    t0 = time.time()
+    dvoff, doff, dadj = T.batched_graph_device(n, ef, seeds)
+    nnz = dadj.n
+    log(f"[bench] batched: {len(n)} graphs, {int(n.sum())} vertices, nnz={nnz} "
+        f"built in {time.time() - t0:.1f}s")
+    pg = T.DeviceArray(len(n), np.uint64)
+    ref = {}
+
+    def step():
+        st = T.batched_count_device(dvoff, doff, dadj, pg)
+        if ref.setdefault("tri", st["triangles"]) != st["triangles"]:
+            raise RuntimeError("batched count changed between steps")
+        return st
+
+    ms, stats = timed_steps(T, step, args.steps, args.warmup)
+    per_graph = pg.to_host()
+    st = min(stats, key=lambda s: s["ms_total"])
+    return {"metric": "batched TC edges/sec over 4096 graphs", "value": nnz / (ms * 1e-3),
+            "unit": "edges/s", "ms_per_step": ms, "higher_is_better": True, "dtype": "int32",
+            "config": {"workload": "batched static TC, 4096 R-MAT graphs, n log-uniform "
+                                   "[1k,64k], ef U{4..32}, seeds 1000+i",
+                       "nnz": nnz, "sum_per_graph": int(per_graph.sum()),
+                       "bins_ms": st["ms_bin"], "ms_prepare": st["ms_prepare"]},
+            "roofline": None, "cpu_baseline": None}
+
+
+def bench_copy(T, args) -> dict:
+    """A plain device copy of known size (1 GiB); its FETCH_SIZE / WRITE_SIZE
+    readings calibrate the traffic counters of the count kernel."""
+    import torch
+
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+
+    def step():
+        y.copy_(x)
+
+    ms, _ = timed_steps(torch.cuda, step, args.steps, args.warmup)
+    return {"metric": "device copy GB/s (traffic calibration)", "value": 2 * 4 * n / ms / 1e6,
+            "unit": "GB/s", "ms_per_step": ms, "higher_is_better": True, "dtype": "int32",
+            "config": {"workload": f"copy of {4 * n} bytes"}, "roofline": None,
+            "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="static", choices=["static", "stream", "batched", "copy"])
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--ef", type=int, default=32)
     ap.add_argument("--seed", type=int, default=4)
@@ -77,76 +330,17 @@ def main() -> int:
 
     import custingeralg_amd as T
 
-    log(f"[bench] device {T.device_name()}")
-    t0 = time.time()
-    g = T.rmat_graph_device(args.scale, args.ef, args.seed)
-    info = g.info()
-    nnz = info["nnz"]
-    log(f"[bench] R-MAT scale {args.scale} ef {args.ef} seed {args.seed}: nnz={nnz} "
-        f"max_degree={info['max_degree']} built in {time.time() - t0:.1f}s")
-
-    def run(policy):
-        sched = T.default_schedule(policy=policy)
-        pv, glob, st = T.static_count(g, sched, per_vertex=True)
-        return pv, glob, st
-
-    fixed_pv, fixed_glob, _ = run(T.POLICY_FIXED)
-    for _ in range(args.warmup):
-        run(T.POLICY_FIXED)
-    T.synchronize()
-    times, stats = [], []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        _, glob, st = run(T.POLICY_FIXED)
-        times.append(time.perf_counter() - ts)
-        stats.append(st)
-        assert glob == fixed_glob, "global count changed between steps"
-        log(f"[bench] step {k}: {times[-1] * 1e3:.1f} ms (events {st['ms_total']:.1f} ms)")
-    T.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ms_step = elapsed / args.steps * 1e3
-
-    # dominant kernel: the bin whose count kernel takes longest (HIP events on its stream)
-    st = min(stats, key=lambda s: s["ms_total"])
-    b = int(np.argmax(st["ms_bin"]))
-    ms_b = st["ms_bin"][b]
-    achieved = st["bytes_alg_bin"][b] / (ms_b * 1e-3) / 1e9 if ms_b > 0 else 0.0
-
-    cpu = None
-    if not args.no_cpu:
-        off, adj = g.to_csr()
-        cpu = cpu_baseline(off, adj, args.cpu_budget)
-        del off, adj
-    g.close()
-
-    rec = {
-        "metric": METRIC,
-        "value": nnz / (ms_step * 1e-3),
-        "unit": "edges/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": f"static TC, R-MAT scale {args.scale} ef {args.ef} "
-                               f"(0.57,0.19,0.19,0.05) seed {args.seed}",
-                   "nnz": nnz, "triangles": fixed_glob // 6, "global_count": fixed_glob,
-                   "policy": "fixed"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_count bin {b}", "kernel_ms": ms_b,
-                     "alg_bytes": st["bytes_alg_bin"][b]},
-        "cpu_baseline": cpu,
-        "bins": {"ms": st["ms_bin"], "tasks": st["tasks_bin"], "jobs": st["jobs_bin"],
-                 "alg_bytes": st["bytes_alg_bin"], "strategy_tasks": st["strategy_tasks"],
-                 "ms_prepare": st["ms_prepare"], "ms_count": st["ms_count"]},
-    }
-    print(json.dumps(rec), flush=True)
+    log(f"[bench] device {T.device_name()}, workload {args.workload}")
+    fn = {"static": bench_static, "stream": bench_stream, "batched": bench_batched,
+          "copy": bench_copy}[args.workload]
+    rec = fn(T, args)
+    line = {"metric": rec.pop("metric"), "value": rec.pop("value"), "unit": rec.pop("unit"),
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": rec.pop("ms_per_step"), "higher_is_better": rec.pop("higher_is_better"),
+            "scaling": "weak", "vs_baseline": None, "dtype": rec.pop("dtype"),
+            "data": "synthetic"}
+    line.update(rec)
+    print(json.dumps(line), flush=True)
     return 0
 
 

@@ -499,6 +499,39 @@ def er_graph_device(n: int, avg_deg: int, seed: int, slack_pct=0) -> cuStinger:
     return _canon(n, m, du, dv, slack_pct)
 
 
+_M64 = (1 << 64) - 1
+
+
+def _mix64(z: int) -> int:
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def rng(seed: int, ctr: int) -> int:
+    """The counter-based generator the device kernels use (keyed splitmix64 finaliser)."""
+    key = _mix64((seed + 0x9E3779B97F4A7C15) & _M64)
+    return _mix64(key ^ ((ctr * 0x9E3779B97F4A7C15 + 0x632BE59BD9B4E019) & _M64))
+
+
+def batched_spec(G: int = 4096, seed_root: int = 1000, nmin: int = 1024, nmax: int = 65536,
+                 efmin: int = 4, efmax: int = 32):
+    """Sizes of a batched workload of G R-MAT graphs: graph i has
+    n_i = floor(exp(U(ln nmin, ln nmax))) vertices and edge factor
+    ef_i ~ U{efmin..efmax}, drawn from rng(seed_root + i, 0 / 1), and seed seed_root + i."""
+    import math
+    n = np.empty(G, np.int32)
+    ef = np.empty(G, np.int32)
+    seeds = np.empty(G, np.uint64)
+    lo, hi = math.log(nmin), math.log(nmax)
+    for i in range(G):
+        u01 = (rng(seed_root + i, 0) >> 11) / float(1 << 53)
+        n[i] = min(nmax, max(nmin, int(math.floor(math.exp(lo + u01 * (hi - lo))))))
+        ef[i] = efmin + rng(seed_root + i, 1) % (efmax - efmin + 1)
+        seeds[i] = seed_root + i
+    return n, ef, seeds
+
+
 def batched_graph_device(n, ef, seeds, abc=(0.57, 0.19, 0.19)):
     """Concatenated CSR of G R-MAT graphs built on the device -> (voff, off, adj) DeviceArrays."""
     n = np.ascontiguousarray(n, np.int32)

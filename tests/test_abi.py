@@ -10,6 +10,7 @@ import re
 import numpy as np
 
 import custingeralg_amd as T
+import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "custinger_tc.h")
@@ -92,3 +93,12 @@ def test_host_to_host_copy():
     assert T.lib().tc_memcpy(b.ctypes.data, a.ctypes.data, a.nbytes, 3) == T.TC_OK
     assert np.array_equal(a, b)
     assert T.lib().tc_memcpy(b.ctypes.data, a.ctypes.data, a.nbytes, 9) == T.TC_ERR_INVALID
+
+
+def test_host_rng_and_batched_spec_match_oracle():
+    """The package's host copy of the counter RNG and the batched workload sizes
+    agree with the oracle's C generator."""
+    for seed, ctr in [(0, 0), (1, 2), (1000, 1), (2**63, 2**40 + 7)]:
+        assert T.rng(seed, ctr) == O.rng(seed, ctr)
+    for a, b in zip(T.batched_spec(G=300), O.batched_spec(G=300)):
+        assert np.array_equal(a, b)
