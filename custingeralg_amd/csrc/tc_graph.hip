@@ -424,9 +424,11 @@ void graph_destroy(Graph *g)
     DevBuf *bufs[] = {&g->st.boff, &g->st.len, &g->st.cap, &g->st.pool, &g->vinfo, &g->padj,
                       &g->coff, &g->crow, &g->work, &g->jobs, &g->tmp, &g->tmp2, &g->tmp3, &g->tmp4,
                       &g->accum, &g->bsrc, &g->bdst, &g->bflag, &g->bkeys, &g->bkeys2,
-                      &g->bvals2, &g->boffb, &g->bdel, &g->flagpool, &g->vmark};
+                      &g->bvals2, &g->boffb, &g->bdel, &g->flagpool, &g->vmark, &g->order,
+                      &g->rnk, &g->tpos};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : g->sb) b.release();
+    for (DevBuf &b : g->ov) b.release();
     if (g->events_ready)
         for (auto &e : g->ev) (void)hipEventDestroy(e);
     delete g;

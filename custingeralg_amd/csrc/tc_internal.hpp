@@ -112,6 +112,12 @@ struct Graph {
     int32_t max_degree = 0;
     // cached derived structures (rebuilt after updates)
     DevBuf vinfo, padj, coff, crow, work, jobs, tmp, tmp2, tmp3, tmp4, accum;
+    // oriented layout in rank space: order[r] = vertex of rank r, rnk = its
+    // inverse, tpos[e] = for a task entry e (a in N-(b)), the position of b
+    // in the rank-sorted N+(a); ov = scratch of the orientation
+    DevBuf order, rnk, tpos;
+    DevBuf ov[8];
+    bool oriented_views = false;
     DevBuf bsrc, bdst, bflag, bkeys, bkeys2, bvals2, boffb, bdel;
     // streaming: per-entry "in the current batch" flags parallel to the pool,
     // per-vertex marks, and batch scratch

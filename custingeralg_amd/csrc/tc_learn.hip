@@ -264,10 +264,13 @@ __global__ void k_model_table(const uint16_t *__restrict__ wb, int small_max, in
 
 // --------------------------------------------------------------- timing --
 
+// Samples task entries of the oriented layout: owner (rank), task vertex
+// (rank) and the start of the task's useful suffix of N+(task).
 __global__ void k_sample_tasks(const int64_t *__restrict__ coff, const VInfo *__restrict__ vi,
-                               const int32_t *__restrict__ padj, int32_t nv, int64_t nnz,
-                               int64_t ncand, uint64_t seed, int32_t lmax,
-                               int32_t *__restrict__ so, int32_t *__restrict__ st)
+                               const int32_t *__restrict__ padj, const int32_t *__restrict__ tpos,
+                               int32_t nv, int64_t nnz, int64_t ncand, uint64_t seed, int32_t lmax,
+                               int32_t *__restrict__ so, int32_t *__restrict__ st,
+                               int32_t *__restrict__ sk)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ncand;
          s += (int64_t)gridDim.x * blockDim.x) {
@@ -283,18 +286,21 @@ __global__ void k_sample_tasks(const int64_t *__restrict__ coff, const VInfo *__
         }
         const VInfo vb = vi[lo];
         const int64_t k = e - coff[lo];
-        int32_t own = -1, task = -1;
+        int32_t own = -1, task = -1, kp = 0;
         const int32_t L = vb.len - vb.dm;
         if (k < vb.dm && L >= 1 && L <= lmax) {
             const int32_t a = padj[vb.rb + k];
+            const int32_t p = tpos[vb.rb + k];
             const VInfo va = vi[a];
-            if (va.len - va.dm >= 1) {
+            if (va.len - va.dm - p - 1 >= 1) {
                 own = lo;
                 task = a;
+                kp = p;
             }
         }
         so[s] = own;
         st[s] = task;
+        sk[s] = kp;
     }
 }
 
@@ -351,7 +357,8 @@ constexpr int kTimeLmax = 4096;
 // One wavefront per sample: the owner's N+ list goes into an LDS table, then
 // the three strategies run back to back, each bracketed by the shader clock.
 __global__ __launch_bounds__(64) void k_time_tasks(const int32_t *__restrict__ so,
-                                                   const int32_t *__restrict__ st, int64_t n,
+                                                   const int32_t *__restrict__ st,
+                                                   const int32_t *__restrict__ sk, int64_t n,
                                                    const VInfo *__restrict__ vi,
                                                    const int32_t *__restrict__ padj,
                                                    float *__restrict__ cost,
@@ -362,11 +369,11 @@ __global__ __launch_bounds__(64) void k_time_tasks(const int32_t *__restrict__ s
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     (void)lt;
     for (int64_t s = blockIdx.x; s < n; s += gridDim.x) {
-        const int32_t b = so[s], a = st[s];
+        const int32_t b = so[s], a = st[s], kp = sk[s];
         const VInfo vb = vi[b], va = vi[a];
-        const int32_t L = vb.len - vb.dm, pl = va.len - va.dm;
+        const int32_t L = vb.len - vb.dm, pl = va.len - va.dm - kp - 1;
         const int32_t *S = padj + vb.rb + vb.dm;
-        const int32_t *T = padj + va.rb + va.dm;
+        const int32_t *T = padj + va.rb + va.dm + kp + 1;
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;
         const Tbl tb{HB, nbb};
@@ -464,22 +471,27 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
     TC_TRY(prepare_views(g, true));
     hipStream_t s = g.stream;
     const int64_t ncand = 4 * max_samples;
-    DevBuf so, st, cst, agr;
+    DevBuf so, st, sk, cst, agr;
     auto done = [&](int r) {
         so.release();
         st.release();
+        sk.release();
         cst.release();
         agr.release();
         return r;
     };
     int r;
-    if ((r = so.ensure((size_t)ncand * 4)) || (r = st.ensure((size_t)ncand * 4))) return done(r);
+    if ((r = so.ensure((size_t)ncand * 4)) || (r = st.ensure((size_t)ncand * 4)) ||
+        (r = sk.ensure((size_t)ncand * 4)))
+        return done(r);
     hipLaunchKernelGGL(k_sample_tasks, dim3(grid_for(ncand, 256, 4096)), dim3(256), 0, s,
-                       g.coff.as<int64_t>(), g.vinfo.as<VInfo>(), g.padj.as<int32_t>(), g.st.nv,
-                       g.st.nnz, ncand, seed, kTimeLmax, so.as<int32_t>(), st.as<int32_t>());
-    std::vector<int32_t> ho(ncand), ht(ncand);
+                       g.coff.as<int64_t>(), g.vinfo.as<VInfo>(), g.padj.as<int32_t>(),
+                       g.tpos.as<int32_t>(), g.st.nv, g.st.nnz, ncand, seed, kTimeLmax,
+                       so.as<int32_t>(), st.as<int32_t>(), sk.as<int32_t>());
+    std::vector<int32_t> ho(ncand), ht(ncand), hk(ncand);
     if (hipMemcpy(ho.data(), so.p, (size_t)ncand * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(ht.data(), st.p, (size_t)ncand * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        hipMemcpy(ht.data(), st.p, (size_t)ncand * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hk.data(), sk.p, (size_t)ncand * 4, hipMemcpyDeviceToHost) != hipSuccess) {
         set_last_error(TC_ERR_HIP, "collect_timings: sample readback");
         return done(TC_ERR_HIP);
     }
@@ -488,13 +500,15 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
         if (ho[i] >= 0) {
             ho[n] = ho[i];
             ht[n] = ht[i];
+            hk[n] = hk[i];
             ++n;
         }
     if (n == 0) return done(TC_OK);
     if ((r = cst.ensure((size_t)n * 3 * sizeof(float))) || (r = agr.ensure((size_t)n * 4)))
         return done(r);
     if (hipMemcpy(so.p, ho.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(st.p, ht.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(st.p, ht.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(sk.p, hk.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) {
         set_last_error(TC_ERR_HIP, "collect_timings: upload");
         return done(TC_ERR_HIP);
     }
@@ -502,7 +516,8 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipLaunchKernelGGL(k_time_tasks, dim3((unsigned)std::min<int64_t>(n, (int64_t)cus * 4)), dim3(64),
-                       0, s, so.as<int32_t>(), st.as<int32_t>(), n, g.vinfo.as<VInfo>(),
+                       0, s, so.as<int32_t>(), st.as<int32_t>(), sk.as<int32_t>(), n,
+                       g.vinfo.as<VInfo>(),
                        g.padj.as<int32_t>(), cst.as<float>(), agr.as<int32_t>());
     std::vector<float> hc((size_t)n * 3);
     std::vector<int32_t> ha((size_t)n);
@@ -523,7 +538,7 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
     const tc_schedule d = default_schedule();
     for (int64_t i = 0; i < n; ++i) {
         const VInfo vb = hv[ho[i]], va = hv[ht[i]];
-        const float L = (float)(vb.len - vb.dm), pl = (float)(va.len - va.dm);
+        const float L = (float)(vb.len - vb.dm), pl = (float)(va.len - va.dm - hk[i] - 1);
         const float fl = std::log2(L), fp = std::log2(pl);
         const int bin = L <= d.small_max ? 0 : (L <= d.mid_max ? 1 : 2);
         feat[i * 4 + 0] = fl / 16.f;

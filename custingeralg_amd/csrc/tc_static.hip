@@ -245,33 +245,66 @@ __device__ inline int64_t plus_prefix(const int64_t *__restrict__ cscan,
     return cscan[c] + __popcll(m);
 }
 
-__global__ void k_orient_vinfo(const int64_t *__restrict__ coff, const int32_t *__restrict__ len,
-                               const int64_t *__restrict__ cscan, const uint64_t *__restrict__ mask,
-                               int32_t nv, VInfo *__restrict__ vi, int32_t *__restrict__ maxplus)
+// dmo[v] = |N-(v)| of vertex v (store order).
+__global__ void k_orient_dm(const int64_t *__restrict__ coff, const int32_t *__restrict__ len,
+                            const int64_t *__restrict__ cscan, const uint64_t *__restrict__ mask,
+                            int32_t nv, int32_t *__restrict__ dmo)
 {
-    int32_t mp = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x) {
-        int64_t p0 = plus_prefix(cscan, mask, coff[v]);
-        int64_t p1 = plus_prefix(cscan, mask, coff[v + 1]);
-        int32_t dp = (int32_t)(p1 - p0);
-        VInfo x;
-        x.rb = coff[v];
-        x.len = len[v];
-        x.dm = x.len - dp;
-        vi[v] = x;
-        mp = max(mp, dp);
+        const int64_t p0 = plus_prefix(cscan, mask, coff[v]);
+        const int64_t p1 = plus_prefix(cscan, mask, coff[v + 1]);
+        dmo[v] = len[v] - (int32_t)(p1 - p0);
     }
-    mp = max(mp, __shfl_xor(mp, 1));
-    for (int d = 2; d < kWave; d <<= 1) mp = max(mp, __shfl_xor(mp, d));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxplus, mp);
 }
 
-__global__ void k_orient_write(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
-                               const int64_t *__restrict__ boff,
-                               const int32_t *__restrict__ pool, const int64_t *__restrict__ cscan,
-                               const uint64_t *__restrict__ mask, const VInfo *__restrict__ vi,
-                               int32_t nv, int64_t nnz, int32_t *__restrict__ padj)
+// Sort keys of the rank order: the degree, with the vertex id as payload (the
+// LSD radix sort is stable, so ties stay in id order: rank = (degree, id)).
+__global__ void k_rank_keys(const int32_t *__restrict__ len, int32_t nv, uint64_t *__restrict__ keys,
+                            int32_t *__restrict__ ids)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        keys[v] = (uint64_t)(uint32_t)len[v];
+        ids[v] = (int32_t)v;
+    }
+}
+
+// rnk[order[r]] = r and the row lengths in rank order.
+__global__ void k_rank_of(const int32_t *__restrict__ order, const int32_t *__restrict__ len,
+                          int32_t nv, int32_t *__restrict__ rnk, int32_t *__restrict__ lr)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = order[r];
+        rnk[v] = (int32_t)r;
+        lr[r] = len[v];
+    }
+}
+
+// Rank-space views: row r of the oriented layout starts at rcoff[r].
+__global__ void k_rank_vinfo(const int64_t *__restrict__ rcoff, const int32_t *__restrict__ order,
+                             const int32_t *__restrict__ dmo, const int32_t *__restrict__ lr,
+                             int32_t nv, VInfo *__restrict__ vi)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        VInfo x;
+        x.rb = rcoff[r];
+        x.dm = dmo[order[r]];
+        x.len = lr[r];
+        vi[r] = x;
+    }
+}
+
+// Writes the N+ part of every row into the rank layout, as ranks (unsorted
+// yet); the N- parts are filled by k_orient_transpose.
+__global__ void k_orient_write_plus(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
+                                    const int64_t *__restrict__ boff, const int32_t *__restrict__ pool,
+                                    const int64_t *__restrict__ cscan,
+                                    const uint64_t *__restrict__ mask, const int32_t *__restrict__ dmo,
+                                    const int32_t *__restrict__ rnk, const int64_t *__restrict__ rcoff,
+                                    int32_t nv, int64_t nnz, int32_t *__restrict__ padj)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nchunks = ceil_div(nnz, kWave);
@@ -279,19 +312,146 @@ __global__ void k_orient_write(const int64_t *__restrict__ coff, const int32_t *
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
-        int64_t e = c * kWave + lane;
-        int32_t a = wave_row_of(coff, crow, nv, c, nnz);
+        const int64_t e = c * kWave + lane;
+        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
         if (e >= nnz) continue;
-        uint64_t m = mask[c];
-        bool plus = (m >> lane) & 1ull;
-        int64_t pe = cscan[c] + __popcll(m & lt);
-        int64_t ca = coff[a];
-        int64_t rp = pe - plus_prefix(cscan, mask, ca);
-        int64_t k = e - ca;
-        int32_t b = pool[boff[a] + k];
-        int64_t dst = plus ? (ca + vi[a].dm + rp) : (ca + (k - rp));
-        padj[dst] = b;
+        const uint64_t m = mask[c];
+        if (!((m >> lane) & 1ull)) continue;
+        const int64_t pe = cscan[c] + __popcll(m & lt);
+        const int64_t ca = coff[a];
+        const int64_t rp = pe - plus_prefix(cscan, mask, ca);
+        const int32_t b = pool[boff[a] + (e - ca)];
+        padj[rcoff[rnk[a]] + dmo[a] + rp] = rnk[b];
     }
+}
+
+// Sorts the N+ part of rows with 2..64 entries (one wavefront per row, bitonic
+// network across the lanes) and lists the longer rows by size class:
+// (64,512] -> big[0], (512,4096] -> big[1], (4096,16384] -> big[2], above -> big[3].
+__global__ void k_sort_plus_wave(const VInfo *__restrict__ vi, int32_t nv, int32_t *__restrict__ padj,
+                                 int32_t *__restrict__ big0, int32_t *__restrict__ big1,
+                                 int32_t *__restrict__ big2, int32_t *__restrict__ big3,
+                                 int32_t *__restrict__ nbig /* [4] */)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < nv;
+         r += wstride) {
+        const VInfo x = vi[r];
+        const int32_t dp = x.len - x.dm;
+        if (dp < 2) continue;
+        if (dp > kWave) {
+            if (lane == 0) {
+                const int q = dp <= 512 ? 0 : (dp <= 4096 ? 1 : (dp <= 16384 ? 2 : 3));
+                int32_t *lst = q == 0 ? big0 : (q == 1 ? big1 : (q == 2 ? big2 : big3));
+                lst[atomicAdd(&nbig[q], 1)] = (int32_t)r;
+            }
+            continue;
+        }
+        int32_t *P = padj + x.rb + x.dm;
+        int32_t v = lane < dp ? P[lane] : INT32_MAX;
+#pragma unroll
+        for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int32_t y = __shfl_xor(v, j, kWave);
+                const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+                v = keep_min ? min(v, y) : max(v, y);
+            }
+        }
+        if (lane < dp) P[lane] = v;
+    }
+}
+
+// Bitonic sort in LDS of the N+ part of the listed rows (one block per row,
+// rows of at most CAP entries).
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void k_sort_plus_block(const VInfo *__restrict__ vi,
+                                                        const int32_t *__restrict__ rows,
+                                                        const int32_t *__restrict__ nrows,
+                                                        int32_t *__restrict__ padj)
+{
+    __shared__ int32_t S[CAP];
+    const int32_t n = *nrows;
+    for (int32_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const VInfo x = vi[rows[q]];
+        const int32_t dp = x.len - x.dm;
+        int32_t *P = padj + x.rb + x.dm;
+        int32_t n2 = 1;
+        while (n2 < dp) n2 <<= 1;
+        for (int32_t i = threadIdx.x; i < n2; i += NT) S[i] = i < dp ? P[i] : INT32_MAX;
+        __syncthreads();
+        for (int32_t k = 2; k <= n2; k <<= 1) {
+            for (int32_t j = k >> 1; j > 0; j >>= 1) {
+                for (int32_t i = threadIdx.x; i < (n2 >> 1); i += NT) {
+                    const int32_t a = 2 * i - (i & (j - 1));  // pair (a, a+j): bit j of a is 0
+                    const int32_t b = a + j;
+                    const int32_t x0 = S[a], x1 = S[b];
+                    if ((x0 > x1) == ((a & k) == 0)) {
+                        S[a] = x1;
+                        S[b] = x0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int32_t i = threadIdx.x; i < dp; i += NT) P[i] = S[i];
+        __syncthreads();
+    }
+}
+
+__global__ void k_widen_u32(const int32_t *__restrict__ x, int64_t n, uint64_t *__restrict__ y)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (uint64_t)(uint32_t)x[i];
+}
+
+__global__ void k_narrow_u64(const uint64_t *__restrict__ y, int64_t n, int32_t *__restrict__ x)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (int32_t)y[i];
+}
+
+// Fills the N- parts: every N+ entry (row r, position k, value w) puts r into
+// the N- part of row w together with k, so the task (r in N-(w)) knows where
+// its useful suffix of N+(r) starts.  Order inside an N- part is arbitrary.
+__global__ void k_orient_transpose(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
+                                   const VInfo *__restrict__ vi, int32_t nv, int64_t nnz,
+                                   int32_t *__restrict__ padj, int32_t *__restrict__ tpos,
+                                   int32_t *__restrict__ cursor, int32_t *__restrict__ bad)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        const int64_t e = c * kWave + lane;
+        const int32_t r = wave_row_of(coff, crow, nv, c, nnz);
+        if (e >= nnz) continue;
+        const VInfo x = vi[r];
+        const int64_t k = e - x.rb - x.dm;
+        if (k < 0) continue;
+        const int32_t w = padj[e];
+        const int32_t slot = atomicAdd(&cursor[w], 1);
+        const VInfo xw = vi[w];
+        if (slot < xw.dm) {
+            padj[xw.rb + slot] = r;
+            tpos[xw.rb + slot] = (int32_t)k;
+        } else {
+            atomicOr(bad, 1);  // rows are not symmetric
+        }
+    }
+}
+
+// Every N- part must have been filled exactly (holds for symmetric rows).
+__global__ void k_check_cursor(const int32_t *__restrict__ cursor, const VInfo *__restrict__ vi,
+                               int32_t nv, int32_t *__restrict__ bad)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x)
+        if (cursor[r] != vi[r].dm) atomicOr(bad, 1);
 }
 
 __global__ void k_direct_vinfo(const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
@@ -335,8 +495,9 @@ __device__ inline int owner_bin(int32_t L, const PlanParams &P)
 template <bool ORIENTED>
 __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
                              const VInfo *__restrict__ vi,
-                             const int32_t *__restrict__ adj, int32_t nv, int64_t nnz,
-                             PlanParams P, unsigned long long *__restrict__ work,
+                             const int32_t *__restrict__ adj, const int32_t *__restrict__ tpos,
+                             int32_t nv, int64_t nnz, PlanParams P,
+                             unsigned long long *__restrict__ work,
                              unsigned long long *__restrict__ bstats)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -357,7 +518,8 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
             if (is_task_entry && L > 0) {
                 int32_t a = adj[vb.rb + k];
                 VInfo va = vi[a];
-                int32_t pl = ORIENTED ? va.len - va.dm : va.len;
+                // oriented: only the part of N+(a) ranked above the owner can hit
+                int32_t pl = ORIENTED ? va.len - va.dm - tpos[vb.rb + k] - 1 : va.len;
                 bool ok = ORIENTED ? (pl > 0) : (a != b && va.len >= 2 && vb.len >= 2);
                 if (ok) {
                     w = (uint32_t)est_items(min(L, P.lmax[2]), pl);
@@ -465,6 +627,8 @@ __global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restr
 struct CountArgs {
     const int32_t *adj;
     const VInfo *vi;
+    const int32_t *tpos;   // oriented: suffix start of each task entry
+    const int32_t *order;  // oriented: vertex of each rank (outputs are by vertex)
     uint32_t *out;
     unsigned long long *global;
     unsigned long long *strat;  // [3] tasks per strategy (may be null)
@@ -557,10 +721,12 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             const int32_t t = J.t_begin + (g0 * kWave + lane) * NW + wid;
             bool valid = t < J.t_end;
             int32_t tv = valid ? A.adj[vo.rb + t] : 0;
+            const int32_t kp = (ORIENTED && valid) ? A.tpos[vo.rb + t] : 0;
             VInfo vt = {0, 0, 0};
             if (valid) vt = A.vi[tv];
-            int64_t pb = ORIENTED ? vt.rb + vt.dm : vt.rb;
-            int32_t pl = ORIENTED ? vt.len - vt.dm : vt.len;
+            // oriented: the task list is the suffix of N+(tv) after the owner
+            int64_t pb = ORIENTED ? vt.rb + vt.dm + kp + 1 : vt.rb;
+            int32_t pl = ORIENTED ? vt.len - vt.dm - kp - 1 : vt.len;
             if (!ORIENTED && valid &&
                 (tv == J.owner || vt.len < 2 || vo.len < 2 || (A.skip_lower && tv < J.owner)))
                 valid = false;
@@ -746,7 +912,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             const uint32_t c = tcnt[wid][lane] + mcount;
             if (valid && c) {
                 hits_job += c;
-                if (ORIENTED) atomicAdd(&A.out[tv], 2u * c);
+                if (ORIENTED) atomicAdd(&A.out[A.order[tv]], 2u * c);
             }
             wave_sync();
         }
@@ -759,17 +925,18 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         if (ORIENTED) {
             for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
                 const uint32_t ci = C[i];
-                if (ci) atomicAdd(&A.out[T.id_at(i)], 2u * ci);
+                if (ci) atomicAdd(&A.out[A.order[T.id_at(i)]], 2u * ci);
             }
         }
+        const int32_t owner_v = ORIENTED ? A.order[J.owner] : J.owner;
         if (threadIdx.x == 0 && hjob)
-            atomicAdd(&A.out[J.owner], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
+            atomicAdd(&A.out[owner_v], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
         hits_all += (threadIdx.x == 0) ? hjob : 0ull;
         if (A.per_graph && threadIdx.x == 0 && hjob) {
             int32_t lo = 0, hi = A.ngraphs - 1;  // last graph whose first vertex <= owner
             while (lo < hi) {
                 const int32_t mid = (lo + hi + 1) >> 1;
-                if (A.gvoff[mid] <= J.owner) lo = mid; else hi = mid - 1;
+                if (A.gvoff[mid] <= owner_v) lo = mid; else hi = mid - 1;
             }
             atomicAdd(&A.per_graph[lo], (ORIENTED ? 6ull : 1ull) * hjob);
         }
@@ -854,50 +1021,147 @@ int prepare_views(Graph &g, bool oriented)
     const int32_t nv = g.st.nv;
     const int64_t nnz = g.st.nnz;
     hipStream_t s = g.stream;
-    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
-    unsigned long long *acc = g.accum.as<unsigned long long>();
-    // compact offsets of the store rows
-    TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
-    int64_t *coff = g.coff.as<int64_t>();
-    TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), coff, nv, g.tmp, s));
-
-    TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
-    VInfo *vi = g.vinfo.as<VInfo>();
-    const int32_t *adj = nullptr;
     const int64_t nchunks = ceil_div(nnz, kWave);
-    TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
-    int32_t *crow = g.crow.as<int32_t>();
-    hipLaunchKernelGGL(k_chunk_rows, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff, nv, crow);
     const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
-    if (oriented) {
-        TC_TRY(g.padj.ensure((size_t)nnz * sizeof(int32_t)));
-        // scratch: mask (u64 x nchunks), cnt (i32 x nchunks), cscan (i64 x nchunks+1)
-        size_t need = (size_t)nchunks * 8 + (size_t)nchunks * 4 + (size_t)(nchunks + 1) * 8 + 64;
-        TC_TRY(g.tmp2.ensure(need));
-        uint64_t *mask = g.tmp2.as<uint64_t>();
-        int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
-        int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
-        int32_t *maxplus = reinterpret_cast<int32_t *>(acc + 15);
-        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, coff, crow,
-                           g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
-                           nv, nnz, mask, cnt);
+    const int vblocks = grid_for(nv, 256, 65536);
+    g.oriented_views = false;
+    if (!oriented) {
+        // compact offsets of the store rows, chunk rows, views of the rows as stored
+        TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
+        int64_t *coff = g.coff.as<int64_t>();
+        TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), coff, nv, g.tmp, s));
+        TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
+        hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, coff, nv,
+                           g.crow.as<int32_t>());
         TC_HIP(hipGetLastError());
-        TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
-        hipLaunchKernelGGL(k_orient_vinfo, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, coff,
-                           g.st.len.as<int32_t>(), cscan, mask, nv, vi, maxplus);
-        TC_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_orient_write, dim3(wblocks), dim3(256), 0, s, coff, crow,
-                           g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, vi, nv,
-                           nnz, g.padj.as<int32_t>());
-        TC_HIP(hipGetLastError());
-        adj = g.padj.as<int32_t>();
-    } else {
-        TC_TRY(build_direct_vinfo(g));
-        adj = g.st.pool.as<int32_t>();
+        return build_direct_vinfo(g);
     }
 
-    (void)adj;
-    (void)acc;
+    // ---- store-order pass: which entries point to a higher-rank vertex
+    TC_TRY(g.ov[0].ensure((size_t)(nv + 1) * sizeof(int64_t)));      // store-order offsets
+    TC_TRY(g.ov[1].ensure((size_t)(nchunks + 1) * sizeof(int32_t))); // their chunk rows
+    int64_t *ocoff = g.ov[0].as<int64_t>();
+    int32_t *ocrow = g.ov[1].as<int32_t>();
+    TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), ocoff, nv, g.tmp, s));
+    hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, ocoff, nv, ocrow);
+    // scratch: mask (u64 x nchunks), cscan (i64 x nchunks+1), cnt (i32 x nchunks)
+    TC_TRY(g.tmp2.ensure((size_t)nchunks * 8 + (size_t)(nchunks + 1) * 8 + (size_t)nchunks * 4 + 64));
+    uint64_t *mask = g.tmp2.as<uint64_t>();
+    int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
+    int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
+    hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
+                       g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(), nv,
+                       nnz, mask, cnt);
+    TC_HIP(hipGetLastError());
+    TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
+    TC_TRY(g.ov[2].ensure((size_t)nv * sizeof(int32_t)));  // |N-(v)| in store order
+    int32_t *dmo = g.ov[2].as<int32_t>();
+    hipLaunchKernelGGL(k_orient_dm, dim3(vblocks), dim3(256), 0, s, ocoff, g.st.len.as<int32_t>(),
+                       cscan, mask, nv, dmo);
+    TC_HIP(hipGetLastError());
+
+    // ---- rank order: stable sort of the degrees
+    TC_TRY(g.order.ensure((size_t)nv * sizeof(int32_t)));
+    TC_TRY(g.rnk.ensure((size_t)nv * sizeof(int32_t)));
+    TC_TRY(g.ov[3].ensure((size_t)nv * sizeof(uint64_t)));
+    int32_t *order = g.order.as<int32_t>();
+    int32_t *rnk = g.rnk.as<int32_t>();
+    uint64_t *keys = g.ov[3].as<uint64_t>();
+    hipLaunchKernelGGL(k_rank_keys, dim3(vblocks), dim3(256), 0, s, g.st.len.as<int32_t>(), nv, keys,
+                       order);
+    TC_HIP(hipGetLastError());
+    // all 32 bits: the cached maximum degree may predate batch updates
+    TC_TRY(radix_sort_u64(keys, order, nv, 32, g.ov[4], g.ov[5], g.tmp, s));
+    int32_t *lr = reinterpret_cast<int32_t *>(keys);  // keys are no longer needed
+    hipLaunchKernelGGL(k_rank_of, dim3(vblocks), dim3(256), 0, s, order, g.st.len.as<int32_t>(), nv,
+                       rnk, lr);
+    TC_HIP(hipGetLastError());
+
+    // ---- rank layout: offsets, views, N+ parts
+    TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
+    int64_t *coff = g.coff.as<int64_t>();
+    TC_TRY(scan_counts_i32(lr, coff, nv, g.tmp, s));
+    TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
+    VInfo *vi = g.vinfo.as<VInfo>();
+    hipLaunchKernelGGL(k_rank_vinfo, dim3(vblocks), dim3(256), 0, s, coff, order, dmo, lr, nv, vi);
+    TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
+    int32_t *crow = g.crow.as<int32_t>();
+    hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, coff, nv, crow);
+    TC_TRY(g.padj.ensure((size_t)nnz * sizeof(int32_t)));
+    TC_TRY(g.tpos.ensure((size_t)nnz * sizeof(int32_t)));
+    int32_t *padj = g.padj.as<int32_t>();
+    hipLaunchKernelGGL(k_orient_write_plus, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
+                       g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, dmo, rnk, coff,
+                       nv, nnz, padj);
+    TC_HIP(hipGetLastError());
+
+    // ---- sort every N+ part by rank
+    const int64_t cap1 = std::max<int64_t>(1, nnz / 512 + 1);
+    const int64_t cap2 = std::max<int64_t>(1, nnz / 4096 + 1);
+    const int64_t cap3 = std::max<int64_t>(1, nnz / 16384 + 1);
+    TC_TRY(g.ov[6].ensure((size_t)(nv + cap1 + cap2 + cap3 + 4) * sizeof(int32_t)));
+    int32_t *nbig = g.ov[6].as<int32_t>();
+    int32_t *big0 = nbig + 4;
+    int32_t *big1 = big0 + nv;
+    int32_t *big2 = big1 + cap1;
+    int32_t *big3 = big2 + cap2;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(nbig),
+                       (int64_t)4, 0u);
+    hipLaunchKernelGGL(k_sort_plus_wave, dim3(grid_for(nv, 4, 1 << 16)), dim3(256), 0, s, vi, nv,
+                       padj, big0, big1, big2, big3, nbig);
+    TC_HIP(hipGetLastError());
+    int dev = 0, cus = 0;
+    TC_HIP(hipGetDevice(&dev));
+    TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL((k_sort_plus_block<512, 64>), dim3(cus * 16), dim3(64), 0, s, vi, big0,
+                       nbig + 0, padj);
+    hipLaunchKernelGGL((k_sort_plus_block<4096, 256>), dim3(cus * 4), dim3(256), 0, s, vi, big1,
+                       nbig + 1, padj);
+    hipLaunchKernelGGL((k_sort_plus_block<16384, 1024>), dim3(cus), dim3(1024), 0, s, vi, big2,
+                       nbig + 2, padj);
+    TC_HIP(hipGetLastError());
+    int32_t nhuge = 0;
+    TC_HIP(hipMemcpyAsync(&nhuge, nbig + 3, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    if (nhuge > 0) {
+        // rows longer than any LDS tile: radix sort each one in global memory
+        std::vector<int32_t> rows((size_t)nhuge);
+        TC_HIP(hipMemcpy(rows.data(), big3, (size_t)nhuge * sizeof(int32_t), hipMemcpyDeviceToHost));
+        const int vbits = std::max(1, ceil_log2_i((int64_t)nv));
+        for (int32_t r : rows) {
+            VInfo x;
+            TC_HIP(hipMemcpy(&x, vi + r, sizeof(VInfo), hipMemcpyDeviceToHost));
+            const int64_t dp = x.len - x.dm;
+            TC_TRY(g.ov[7].ensure((size_t)dp * sizeof(uint64_t)));
+            uint64_t *k64 = g.ov[7].as<uint64_t>();
+            int32_t *P = padj + x.rb + x.dm;
+            hipLaunchKernelGGL(k_widen_u32, dim3(grid_for(dp, 256, 4096)), dim3(256), 0, s, P, dp, k64);
+            TC_TRY(radix_sort_u64(k64, nullptr, dp, vbits, g.ov[4], g.ov[5], g.tmp, s));
+            hipLaunchKernelGGL(k_narrow_u64, dim3(grid_for(dp, 256, 4096)), dim3(256), 0, s, k64, dp, P);
+            TC_HIP(hipGetLastError());
+        }
+    }
+
+    // ---- N- parts with the suffix starts
+    int32_t *cursor = dmo;  // store-order |N-| is not needed any more
+    int32_t *bad = nbig + 3;
+    hipLaunchKernelGGL(k_fill_u32, dim3(vblocks), dim3(256), 0, s, reinterpret_cast<uint32_t *>(cursor),
+                       (int64_t)nv, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(bad),
+                       (int64_t)1, 0u);
+    hipLaunchKernelGGL(k_orient_transpose, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, nv, nnz,
+                       padj, g.tpos.as<int32_t>(), cursor, bad);
+    hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, nv, bad);
+    TC_HIP(hipGetLastError());
+    int32_t hbad = 0;
+    TC_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    if (hbad) {
+        g.symmetric = 0;
+        set_last_error(TC_ERR_INVALID, "oriented layout: the rows are not symmetric");
+        return TC_ERR_INVALID;
+    }
+    g.oriented_views = true;
     return TC_OK;
 }
 
@@ -925,8 +1189,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         }
         method = ok ? TC_METHOD_ORIENTED : TC_METHOD_DIRECT;
     }
-    const bool oriented = method == TC_METHOD_ORIENTED;
-    st.method_used = method;
+    bool oriented = method == TC_METHOD_ORIENTED;
 
     // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin stats
     TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
@@ -947,12 +1210,24 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         TC_HIP(hipEventRecord(g.ev[1], s));
         TC_HIP(hipEventSynchronize(g.ev[1]));
         if (global) *global = 0;
+        st.method_used = method;
         if (stats) *stats = st;
         return TC_OK;
     }
 
-    // compact offsets, chunk rows and the (degree-partitioned) vertex views
-    TC_TRY(prepare_views(g, oriented));
+    // compact offsets, chunk rows and the (degree-partitioned) vertex views;
+    // rows declared symmetric that turn out not to be go the direct way
+    if (oriented) {
+        const int pv = prepare_views(g, true);
+        if (pv == TC_ERR_INVALID && sched.method == TC_METHOD_AUTO && !g.symmetric) {
+            oriented = false;
+            method = TC_METHOD_DIRECT;
+        } else if (pv != TC_OK) {
+            return pv;
+        }
+    }
+    if (!oriented) TC_TRY(prepare_views(g, false));
+    st.method_used = method;
     int64_t *coff = g.coff.as<int64_t>();
     int32_t *crow = g.crow.as<int32_t>();
     VInfo *vi = g.vinfo.as<VInfo>();
@@ -976,11 +1251,11 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     unsigned long long *work = g.work.as<unsigned long long>();
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, work, (int64_t)nv);
     if (oriented)
-        hipLaunchKernelGGL(k_owner_work<true>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj, nv,
-                           nnz, P, work, stats ? acc + 4 : nullptr);
+        hipLaunchKernelGGL(k_owner_work<true>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj,
+                           g.tpos.as<int32_t>(), nv, nnz, P, work, stats ? acc + 4 : nullptr);
     else
-        hipLaunchKernelGGL(k_owner_work<false>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj, nv,
-                           nnz, P, work, stats ? acc + 4 : nullptr);
+        hipLaunchKernelGGL(k_owner_work<false>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj,
+                           nullptr, nv, nnz, P, work, stats ? acc + 4 : nullptr);
     TC_HIP(hipGetLastError());
     TC_TRY(g.tmp3.ensure((size_t)3 * nv * sizeof(int32_t) + (size_t)3 * (nv + 1) * sizeof(int64_t) + 16));
     int32_t *njobs = g.tmp3.as<int32_t>();
@@ -1016,6 +1291,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     CountArgs A;
     A.adj = adj;
     A.vi = vi;
+    A.tpos = oriented ? g.tpos.as<int32_t>() : nullptr;
+    A.order = oriented ? g.order.as<int32_t>() : nullptr;
     A.out = outp;
     A.global = acc;
     A.strat = acc + 1;
