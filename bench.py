@@ -128,23 +128,33 @@ def bench_static(T, args) -> dict:
             f"bins {[round(x, 1) for x in best['ms_bin']]})")
         return {"ms_per_step": ms, "stats": best}
 
+    Lm.set_default_schedule(None)
     legs = {"fixed": leg("fixed", T.POLICY_FIXED)}
 
-    # learned selector: timings from smaller R-MAT graphs, trained on the device
+    # learned selector and tuned schedule: saved by custingeralg_amd.train_selector
+    # (trained on the GPU from R-MAT 18/20/22 timings), else trained here
     t0 = time.time()
-    train = [T.rmat_graph_device(s, 16, 100 + s) for s in (18, 20, 22)]
-    _, table, losses, (feat, _) = Lm.train_selector(train, samples_per_graph=4096, steps=400)
-    tuned, tune_ms = Lm.autotune(train[-1])
-    for tg in train:
-        tg.close()
+    if os.path.exists(args.selector):
+        sel = Lm.load_selector(args.selector)
+        table, tuned, meta = sel["table"], sel["schedule"], sel["meta"]
+        source = (f"{os.path.relpath(args.selector, ROOT)}: trained on R-MAT "
+                  f"{meta.get('train_scales')} ({meta.get('timed_tasks')} timed tasks), "
+                  f"tuned on {meta.get('tune_graph')}")
+    else:
+        train = [T.rmat_graph_device(s, 16, 100 + s) for s in (18, 20, 22)]
+        _, table, losses, (feat, _) = Lm.train_selector(train, samples_per_graph=4096, steps=400)
+        tuned, _ = Lm.autotune(train[-1])
+        for tg in train:
+            tg.close()
+        source = f"trained in this run on R-MAT 18/20/22 ({len(feat)} timed tasks), tuned on R-MAT 22"
     picks = np.bincount(table.ravel(), minlength=3).tolist()
-    log(f"[bench] selector: {len(feat)} timed tasks, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
-        f"table picks merge/bsearch/hash {picks}; autotune small={tuned.small_max} "
-        f"mid={tuned.mid_max} flush={tuned.flush_items} job={tuned.job_items} "
-        f"({time.time() - t0:.1f}s)")
+    log(f"[bench] selector {source}; table picks merge/bsearch/hash {picks}; schedule "
+        f"small={tuned.small_max} mid={tuned.mid_max} flush={tuned.flush_items} "
+        f"job={tuned.job_items} ({time.time() - t0:.1f}s)")
     legs["tuned"] = leg("tuned", T.POLICY_FIXED)
     legs["learned"] = leg("learned", T.POLICY_LEARNED)
     Lm.unload()
+    Lm.set_default_schedule(None)
 
     main = legs["learned"]
     st = main["stats"]
@@ -166,7 +176,7 @@ def bench_static(T, args) -> dict:
         "config": {"workload": f"static TC, R-MAT scale {args.scale} ef {args.ef} "
                                f"(0.57,0.19,0.19,0.05) seed {args.seed}",
                    "nnz": nnz, "triangles": glob // 6, "global_count": glob,
-                   "schedule": "autotuned + learned strategy table",
+                   "schedule": "autotuned + learned strategy table", "selector": source,
                    "legs_ms_per_step": {k: v["ms_per_step"] for k, v in legs.items()},
                    "learned_vs_fixed": legs["fixed"]["ms_per_step"] / main["ms_per_step"],
                    "tuned": {"small_max": tuned.small_max, "mid_max": tuned.mid_max,
@@ -293,21 +303,23 @@ def bench_batched(T, args) -> dict:
 
 
 def bench_copy(T, args) -> dict:
-    """A plain device copy of known size (1 GiB); its FETCH_SIZE / WRITE_SIZE
-    readings calibrate the traffic counters of the count kernel."""
-    import torch
-
-    n = 1 << 28
-    x = torch.ones(n, dtype=torch.int32, device="cuda")
-    y = torch.empty_like(x)
+    """A plain device-to-device copy of known size (1 GiB, the runtime's copy
+    kernel); its FETCH_SIZE / WRITE_SIZE readings calibrate the traffic
+    counters of the count kernel."""
+    nbytes = 1 << 30
+    src = T.DeviceArray(nbytes // 4, np.int32)
+    dst = T.DeviceArray(nbytes // 4, np.int32)
+    T.check(T.lib().tc_memset(src.ptr, 1, nbytes))
 
     def step():
-        y.copy_(x)
+        T.check(T.lib().tc_memcpy(dst.ptr, src.ptr, nbytes, 2))
 
-    ms, _ = timed_steps(torch.cuda, step, args.steps, args.warmup)
-    return {"metric": "device copy GB/s (traffic calibration)", "value": 2 * 4 * n / ms / 1e6,
+    ms, _ = timed_steps(T, step, args.steps, args.warmup)
+    src.free()
+    dst.free()
+    return {"metric": "device copy GB/s (traffic calibration)", "value": 2 * nbytes / ms / 1e6,
             "unit": "GB/s", "ms_per_step": ms, "higher_is_better": True, "dtype": "int32",
-            "config": {"workload": f"copy of {4 * n} bytes"}, "roofline": None,
+            "config": {"workload": f"device-to-device copy of {nbytes} bytes"}, "roofline": None,
             "cpu_baseline": None}
 
 
@@ -322,6 +334,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--selector", default=os.path.join(ROOT, "custingeralg_amd",
+                                                       "selector_model.json"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "

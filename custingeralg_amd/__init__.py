@@ -45,7 +45,7 @@ EXPORTS = [
     "tc_stream_delta_passes",
     "tc_batched_count", "tc_lb_partition", "tc_lb_degree_bins", "tc_gen_rmat_pairs",
     "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
-    "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune",
+    "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune", "tc_schedule_set_default",
 ]
 
 
@@ -67,13 +67,15 @@ class Stats(C.Structure):
                 ("ms_bin", C.c_double * 4), ("bytes_alg_bin", C.c_double * 4),
                 ("tasks_bin", C.c_int64 * 4), ("jobs_bin", C.c_int64 * 4),
                 ("strategy_tasks", C.c_int64 * 3), ("triangles", C.c_uint64),
-                ("method_used", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("method_used", C.c_int32), ("reserved0", C.c_int32),
+                ("items_bin", C.c_double * 3)]
 
     def as_dict(self) -> dict:
         return {
             "ms_total": self.ms_total, "ms_prepare": self.ms_prepare, "ms_count": self.ms_count,
             "ms_bin": list(self.ms_bin)[:3], "bytes_alg_bin": list(self.bytes_alg_bin)[:3],
             "tasks_bin": list(self.tasks_bin)[:3], "jobs_bin": list(self.jobs_bin)[:3],
+            "items_bin": list(self.items_bin),
             "strategy_tasks": dict(zip(("merge", "bsearch", "hash"), self.strategy_tasks)),
             "triangles": int(self.triangles), "method_used": int(self.method_used),
         }
@@ -145,6 +147,7 @@ def lib():
         L.tc_model_load.argtypes = [vp]
         L.tc_model_table.argtypes = [vp, C.POINTER(i32)]
         L.tc_autotune.argtypes = [vp, C.POINTER(Schedule), C.POINTER(C.c_double)]
+        L.tc_schedule_set_default.argtypes = [C.POINTER(Schedule)]
         _lib = L
     return _lib
 

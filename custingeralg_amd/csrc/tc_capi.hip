@@ -40,6 +40,7 @@ int mlp_forward(const float *params, const float *feat, int64_t n, float *pred);
 int model_load(const float *params_host);
 int model_table(uint8_t *table, int32_t *dim);
 int autotune(Graph &g, tc_schedule *best, double *best_ms);
+int set_default_schedule(const tc_schedule *s);
 bool schedule_override(tc_schedule *s);
 }  // namespace tc
 
@@ -436,6 +437,14 @@ int tc_autotune(tc_graph *g, tc_schedule *best, double *best_ms)
     TC_GUARD({
         TC_CHECK_ARG(g && best, "tc_autotune: bad arguments");
         return tc::autotune(*g->g, best, best_ms);
+    })
+}
+
+int tc_schedule_set_default(const tc_schedule *s)
+{
+    TC_GUARD({
+        TC_TRY(check_schedule(s));
+        return tc::set_default_schedule(s);
     })
 }
 

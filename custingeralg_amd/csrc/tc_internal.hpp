@@ -74,9 +74,10 @@ int radix_sort_u64(uint64_t *keys, int32_t *vals, int64_t n, int bits, DevBuf &a
                    DevBuf &alt_vals, DevBuf &tmp, hipStream_t s);
 
 // ------------------------------------------------------- vertex views --
-// One 16-byte record per vertex used by every intersection kernel:
-// the list of v is adj[rb .. rb+len); in the degree-partitioned layout the
-// first dm entries are N-(v) (lower rank) and the rest N+(v).
+// One 16-byte record per vertex used by every intersection kernel.  Direct
+// layout: the row of v is adj[rb .. rb+len) and dm = 0.  Oriented (rank)
+// layout: len is the degree, dm = |N-(v)| (the tasks of v, kept in a separate
+// task array) and N+(v) = adj[rb .. rb+len-dm), sorted by rank.
 struct VInfo {
     int64_t rb;
     int32_t dm;
@@ -113,10 +114,12 @@ struct Graph {
     // cached derived structures (rebuilt after updates)
     DevBuf vinfo, padj, coff, crow, work, jobs, tmp, tmp2, tmp3, tmp4, accum;
     // oriented layout in rank space: order[r] = vertex of rank r, rnk = its
-    // inverse, tpos[e] = for a task entry e (a in N-(b)), the position of b
-    // in the rank-sorted N+(a); ov = scratch of the orientation
-    DevBuf order, rnk, tpos;
-    DevBuf ov[8];
+    // inverse; tasks = for every owner b (rows by rank, offsets in coff) the
+    // pairs (a, position of b in the rank-sorted N+(a)) for a in N-(b);
+    // ov = scratch of the orientation
+    DevBuf order, rnk, tasks;
+    DevBuf ov[12];
+    int64_t ntasks = 0;  // entries of the task array (oriented layout)
     bool oriented_views = false;
     DevBuf bsrc, bdst, bflag, bkeys, bkeys2, bvals2, boffb, bdel;
     // streaming: per-entry "in the current batch" flags parallel to the pool,

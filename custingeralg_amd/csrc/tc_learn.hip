@@ -264,11 +264,11 @@ __global__ void k_model_table(const uint16_t *__restrict__ wb, int small_max, in
 
 // --------------------------------------------------------------- timing --
 
-// Samples task entries of the oriented layout: owner (rank), task vertex
+// Samples entries of the oriented task array: owner (rank), task vertex
 // (rank) and the start of the task's useful suffix of N+(task).
-__global__ void k_sample_tasks(const int64_t *__restrict__ coff, const VInfo *__restrict__ vi,
-                               const int32_t *__restrict__ padj, const int32_t *__restrict__ tpos,
-                               int32_t nv, int64_t nnz, int64_t ncand, uint64_t seed, int32_t lmax,
+__global__ void k_sample_tasks(const int64_t *__restrict__ toff, const VInfo *__restrict__ vi,
+                               const int2 *__restrict__ tasks, int32_t nv, int64_t ntasks,
+                               int64_t ncand, uint64_t seed, int32_t lmax,
                                int32_t *__restrict__ so, int32_t *__restrict__ st,
                                int32_t *__restrict__ sk)
 {
@@ -278,21 +278,20 @@ __global__ void k_sample_tasks(const int64_t *__restrict__ coff, const VInfo *__
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
         z ^= z >> 31;
-        const int64_t e = (int64_t)((z >> 11) % (uint64_t)nnz);
-        int32_t lo = 0, hi = nv - 1;
+        const int64_t e = (int64_t)((z >> 11) % (uint64_t)ntasks);
+        int32_t lo = 0, hi = nv - 1;  // owner: last row whose first task is <= e
         while (lo < hi) {
             const int32_t mid = lo + ((hi - lo + 1) >> 1);
-            if (coff[mid] <= e) lo = mid; else hi = mid - 1;
+            if (toff[mid] <= e) lo = mid; else hi = mid - 1;
         }
         const VInfo vb = vi[lo];
-        const int64_t k = e - coff[lo];
         int32_t own = -1, task = -1, kp = 0;
         const int32_t L = vb.len - vb.dm;
-        if (k < vb.dm && L >= 1 && L <= lmax) {
-            const int32_t a = padj[vb.rb + k];
-            const int32_t p = tpos[vb.rb + k];
-            const VInfo va = vi[a];
-            if (va.len - va.dm - p - 1 >= 1) {
+        if (L >= 1 && L <= lmax) {
+            const int2 tk = tasks[e];
+            const int32_t a = tk.x;
+            const int32_t p = (vi[a].len - vi[a].dm) - tk.y - 1;  // position of the owner in N+(a)
+            if (tk.y >= 1) {
                 own = lo;
                 task = a;
                 kp = p;
@@ -372,8 +371,8 @@ __global__ __launch_bounds__(64) void k_time_tasks(const int32_t *__restrict__ s
         const int32_t b = so[s], a = st[s], kp = sk[s];
         const VInfo vb = vi[b], va = vi[a];
         const int32_t L = vb.len - vb.dm, pl = va.len - va.dm - kp - 1;
-        const int32_t *S = padj + vb.rb + vb.dm;
-        const int32_t *T = padj + va.rb + va.dm + kp + 1;
+        const int32_t *S = padj + vb.rb;
+        const int32_t *T = padj + va.rb + kp + 1;
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;
         const Tbl tb{HB, nbb};
@@ -469,6 +468,7 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
     }
     if (g.st.nnz == 0) return TC_OK;
     TC_TRY(prepare_views(g, true));
+    if (g.ntasks == 0) return TC_OK;
     hipStream_t s = g.stream;
     const int64_t ncand = 4 * max_samples;
     DevBuf so, st, sk, cst, agr;
@@ -485,9 +485,9 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
         (r = sk.ensure((size_t)ncand * 4)))
         return done(r);
     hipLaunchKernelGGL(k_sample_tasks, dim3(grid_for(ncand, 256, 4096)), dim3(256), 0, s,
-                       g.coff.as<int64_t>(), g.vinfo.as<VInfo>(), g.padj.as<int32_t>(),
-                       g.tpos.as<int32_t>(), g.st.nv, g.st.nnz, ncand, seed, kTimeLmax,
-                       so.as<int32_t>(), st.as<int32_t>(), sk.as<int32_t>());
+                       g.coff.as<int64_t>(), g.vinfo.as<VInfo>(), g.tasks.as<int2>(), g.st.nv,
+                       g.ntasks, ncand, seed, kTimeLmax, so.as<int32_t>(), st.as<int32_t>(),
+                       sk.as<int32_t>());
     std::vector<int32_t> ho(ncand), ht(ncand), hk(ncand);
     if (hipMemcpy(ho.data(), so.p, (size_t)ncand * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(ht.data(), st.p, (size_t)ncand * 4, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -629,6 +629,14 @@ int model_table(uint8_t *table, int32_t *dim)
         return TC_ERR_INVALID;
     }
     TC_HIP(hipMemcpy(table, g_learn.table, kTableDim * kTableDim, hipMemcpyDeviceToHost));
+    return TC_OK;
+}
+
+int set_default_schedule(const tc_schedule *s)
+{
+    std::lock_guard<std::mutex> lk(g_learn_mu);
+    g_learn.has_tuned = s != nullptr;
+    if (s) g_learn.tuned = *s;
     return TC_OK;
 }
 

@@ -112,6 +112,27 @@ struct BucketTable {
     }
 };
 
+// Membership bitmap of a sorted owner tile over its value range [lo, lo+span):
+// bit (w - lo) is set for every tile entry and PF[i] counts the set bits of
+// words < i, so the position of w in the tile (its counter slot) is PF[word]
+// plus the set bits below w in its word.  One 4-byte LDS read per probe.
+struct TileBitmap {
+    const uint32_t *BM;
+    const uint16_t *PF;
+    int32_t lo;
+    uint32_t span;
+
+    __device__ inline int32_t find(int32_t w) const
+    {
+        const uint32_t off = (uint32_t)(w - lo);  // w < lo (and w = -1) wraps past span
+        if (off >= span) return -1;
+        const uint32_t x = BM[off >> 5];
+        const uint32_t bit = off & 31u;
+        if (!((x >> bit) & 1u)) return -1;
+        return (int32_t)PF[off >> 5] + __popc(x & ((1u << bit) - 1u));
+    }
+};
+
 // ------------------------------------------------------------ helpers --
 
 __device__ inline void wave_sync()
@@ -270,40 +291,41 @@ __global__ void k_rank_keys(const int32_t *__restrict__ len, int32_t nv, uint64_
     }
 }
 
-// rnk[order[r]] = r and the row lengths in rank order.
+// rnk[order[r]] = r, and |N+| / |N-| of the vertex of rank r.
 __global__ void k_rank_of(const int32_t *__restrict__ order, const int32_t *__restrict__ len,
-                          int32_t nv, int32_t *__restrict__ rnk, int32_t *__restrict__ lr)
+                          const int32_t *__restrict__ dmo, int32_t nv, int32_t *__restrict__ rnk,
+                          int32_t *__restrict__ dpr, int32_t *__restrict__ dmr)
 {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
          r += (int64_t)gridDim.x * blockDim.x) {
         const int32_t v = order[r];
         rnk[v] = (int32_t)r;
-        lr[r] = len[v];
+        dpr[r] = len[v] - dmo[v];
+        dmr[r] = dmo[v];
     }
 }
 
-// Rank-space views: row r of the oriented layout starts at rcoff[r].
-__global__ void k_rank_vinfo(const int64_t *__restrict__ rcoff, const int32_t *__restrict__ order,
-                             const int32_t *__restrict__ dmo, const int32_t *__restrict__ lr,
-                             int32_t nv, VInfo *__restrict__ vi)
+// Rank-space views: N+ of rank r starts at pcoff[r] of the plus array.
+__global__ void k_rank_vinfo(const int64_t *__restrict__ pcoff, const int32_t *__restrict__ dpr,
+                             const int32_t *__restrict__ dmr, int32_t nv, VInfo *__restrict__ vi)
 {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
          r += (int64_t)gridDim.x * blockDim.x) {
         VInfo x;
-        x.rb = rcoff[r];
-        x.dm = dmo[order[r]];
-        x.len = lr[r];
+        x.rb = pcoff[r];
+        x.dm = dmr[r];
+        x.len = dpr[r] + dmr[r];
         vi[r] = x;
     }
 }
 
-// Writes the N+ part of every row into the rank layout, as ranks (unsorted
-// yet); the N- parts are filled by k_orient_transpose.
+// Writes N+ of every vertex into the plus array of the rank layout, as ranks
+// (unsorted yet); the task array is filled by k_orient_transpose.
 __global__ void k_orient_write_plus(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
                                     const int64_t *__restrict__ boff, const int32_t *__restrict__ pool,
                                     const int64_t *__restrict__ cscan,
-                                    const uint64_t *__restrict__ mask, const int32_t *__restrict__ dmo,
-                                    const int32_t *__restrict__ rnk, const int64_t *__restrict__ rcoff,
+                                    const uint64_t *__restrict__ mask,
+                                    const int32_t *__restrict__ rnk, const int64_t *__restrict__ pcoff,
                                     int32_t nv, int64_t nnz, int32_t *__restrict__ padj)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -321,34 +343,83 @@ __global__ void k_orient_write_plus(const int64_t *__restrict__ coff, const int3
         const int64_t ca = coff[a];
         const int64_t rp = pe - plus_prefix(cscan, mask, ca);
         const int32_t b = pool[boff[a] + (e - ca)];
-        padj[rcoff[rnk[a]] + dmo[a] + rp] = rnk[b];
+        padj[pcoff[rnk[a]] + rp] = rnk[b];
     }
 }
 
-// Sorts the N+ part of rows with 2..64 entries (one wavefront per row, bitonic
-// network across the lanes) and lists the longer rows by size class:
-// (64,512] -> big[0], (512,4096] -> big[1], (4096,16384] -> big[2], above -> big[3].
-__global__ void k_sort_plus_wave(const VInfo *__restrict__ vi, int32_t nv, int32_t *__restrict__ padj,
-                                 int32_t *__restrict__ big0, int32_t *__restrict__ big1,
-                                 int32_t *__restrict__ big2, int32_t *__restrict__ big3,
-                                 int32_t *__restrict__ nbig /* [4] */)
+// One lane per rank row: N+ parts of 2..8 entries are sorted in registers by
+// a bitonic network, longer ones are listed by size class: (8,64] -> lst[0],
+// (64,512] -> lst[1], (512,4096] -> lst[2], (4096,16384] -> lst[3], above ->
+// lst[4].  List q starts at lst + off[q], its length is cnt[q].
+__global__ void k_sort_plus_small(const VInfo *__restrict__ vi, int32_t nv,
+                                  int32_t *__restrict__ padj, int32_t *__restrict__ lst,
+                                  const int64_t *__restrict__ off /* [5] */,
+                                  int32_t *__restrict__ cnt /* [5] */)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < nv;
-         r += wstride) {
-        const VInfo x = vi[r];
-        const int32_t dp = x.len - x.dm;
-        if (dp < 2) continue;
-        if (dp > kWave) {
-            if (lane == 0) {
-                const int q = dp <= 512 ? 0 : (dp <= 4096 ? 1 : (dp <= 16384 ? 2 : 3));
-                int32_t *lst = q == 0 ? big0 : (q == 1 ? big1 : (q == 2 ? big2 : big3));
-                lst[atomicAdd(&nbig[q], 1)] = (int32_t)r;
-            }
-            continue;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // whole waves iterate together so the list appends can be aggregated
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); r0 < nv;
+         r0 += stride) {
+        const int64_t r = r0 + lane;
+        int32_t dp = 0;
+        VInfo x = {0, 0, 0};
+        if (r < nv) {
+            x = vi[r];
+            dp = x.len - x.dm;
         }
-        int32_t *P = padj + x.rb + x.dm;
+        const int q = dp <= 8 ? -1
+                              : (dp <= 64 ? 0 : (dp <= 512 ? 1 : (dp <= 4096 ? 2 : (dp <= 16384 ? 3 : 4))));
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const uint64_t m = __ballot(q == c);
+            if (!m) continue;
+            int32_t base = 0;
+            if (lane == __ffsll((unsigned long long)m) - 1) base = atomicAdd(&cnt[c], __popcll(m));
+            base = __shfl(base, __ffsll((unsigned long long)m) - 1, kWave);
+            if (q == c) lst[off[c] + base + __popcll(m & lt)] = (int32_t)r;
+        }
+        if (dp < 2 || dp > 8) continue;
+        int32_t *P = padj + x.rb;
+        int32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = i < dp ? P[i] : INT32_MAX;
+#pragma unroll
+        for (int k = 2; k <= 8; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const bool up = (i & k) == 0;
+                        const int32_t a = v[i], b = v[l];
+                        v[i] = up ? min(a, b) : max(a, b);
+                        v[l] = up ? max(a, b) : min(a, b);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < dp) P[i] = v[i];
+    }
+}
+
+// N+ parts of 9..64 entries: one wavefront per listed row, bitonic network
+// across the lanes.
+__global__ void k_sort_plus_wave(const VInfo *__restrict__ vi, const int32_t *__restrict__ rows,
+                                 const int32_t *__restrict__ nrows, int32_t *__restrict__ padj)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int32_t n = *nrows;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n;
+         q += wstride) {
+        const VInfo x = vi[rows[q]];
+        const int32_t dp = x.len - x.dm;
+        int32_t *P = padj + x.rb;
         int32_t v = lane < dp ? P[lane] : INT32_MAX;
 #pragma unroll
         for (int k = 2; k <= kWave; k <<= 1) {
@@ -376,7 +447,7 @@ __global__ __launch_bounds__(NT) void k_sort_plus_block(const VInfo *__restrict_
     for (int32_t q = blockIdx.x; q < n; q += gridDim.x) {
         const VInfo x = vi[rows[q]];
         const int32_t dp = x.len - x.dm;
-        int32_t *P = padj + x.rb + x.dm;
+        int32_t *P = padj + x.rb;
         int32_t n2 = 1;
         while (n2 < dp) n2 <<= 1;
         for (int32_t i = threadIdx.x; i < n2; i += NT) S[i] = i < dp ? P[i] : INT32_MAX;
@@ -414,34 +485,30 @@ __global__ void k_narrow_u64(const uint64_t *__restrict__ y, int64_t n, int32_t 
         x[i] = (int32_t)y[i];
 }
 
-// Fills the N- parts: every N+ entry (row r, position k, value w) puts r into
-// the N- part of row w together with k, so the task (r in N-(w)) knows where
-// its useful suffix of N+(r) starts.  Order inside an N- part is arbitrary.
-__global__ void k_orient_transpose(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
-                                   const VInfo *__restrict__ vi, int32_t nv, int64_t nnz,
-                                   int32_t *__restrict__ padj, int32_t *__restrict__ tpos,
-                                   int32_t *__restrict__ cursor, int32_t *__restrict__ bad)
+// Fills the task array: every N+ entry (rank row r, position k, value w)
+// becomes the task (r, |N+(r)| - k - 1) of owner w: the task vertex and the
+// length of the suffix of N+(r) after w, the only part that can hit.  The
+// order of an owner's tasks is arbitrary.
+__global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int32_t *__restrict__ pcrow,
+                                   const int32_t *__restrict__ padj, const VInfo *__restrict__ vi,
+                                   const int64_t *__restrict__ toff, int32_t nv, int64_t nplus,
+                                   int2 *__restrict__ tasks, int32_t *__restrict__ cursor,
+                                   int32_t *__restrict__ bad)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t nchunks = ceil_div(nplus, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         const int64_t e = c * kWave + lane;
-        const int32_t r = wave_row_of(coff, crow, nv, c, nnz);
-        if (e >= nnz) continue;
-        const VInfo x = vi[r];
-        const int64_t k = e - x.rb - x.dm;
-        if (k < 0) continue;
+        const int32_t r = wave_row_of(pcoff, pcrow, nv, c, nplus);
+        if (e >= nplus) continue;
         const int32_t w = padj[e];
         const int32_t slot = atomicAdd(&cursor[w], 1);
-        const VInfo xw = vi[w];
-        if (slot < xw.dm) {
-            padj[xw.rb + slot] = r;
-            tpos[xw.rb + slot] = (int32_t)k;
-        } else {
+        if (slot < vi[w].dm)
+            tasks[toff[w] + slot] = make_int2(r, (int32_t)(pcoff[r + 1] - e - 1));
+        else
             atomicOr(bad, 1);  // rows are not symmetric
-        }
     }
 }
 
@@ -493,9 +560,11 @@ __device__ inline int owner_bin(int32_t L, const PlanParams &P)
 
 // Per-owner estimated work (sum over its tasks); per-bin stats.
 template <bool ORIENTED>
+// Entries are the task array (oriented) or the store rows (direct), with
+// their row offsets coff and chunk rows crow.
 __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
                              const VInfo *__restrict__ vi,
-                             const int32_t *__restrict__ adj, const int32_t *__restrict__ tpos,
+                             const int32_t *__restrict__ adj, const int2 *__restrict__ tasks,
                              int32_t nv, int64_t nnz, PlanParams P,
                              unsigned long long *__restrict__ work,
                              unsigned long long *__restrict__ bstats)
@@ -504,7 +573,7 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
     const int64_t nchunks = ceil_div(nnz, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     // per-lane bin statistics, reduced once at the end
-    unsigned long long st_t[3] = {0, 0, 0}, st_b[3] = {0, 0, 0};
+    unsigned long long st_t[3] = {0, 0, 0}, st_b[3] = {0, 0, 0}, st_i[3] = {0, 0, 0};
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         int64_t e = c * kWave + lane;
@@ -512,15 +581,20 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
         uint32_t w = 0;
         if (e < nnz) {
             VInfo vb = vi[b];
-            int64_t k = e - coff[b];
             int32_t L = ORIENTED ? vb.len - vb.dm : vb.len;
-            bool is_task_entry = ORIENTED ? (k < vb.dm) : true;
-            if (is_task_entry && L > 0) {
-                int32_t a = adj[vb.rb + k];
-                VInfo va = vi[a];
-                // oriented: only the part of N+(a) ranked above the owner can hit
-                int32_t pl = ORIENTED ? va.len - va.dm - tpos[vb.rb + k] - 1 : va.len;
-                bool ok = ORIENTED ? (pl > 0) : (a != b && va.len >= 2 && vb.len >= 2);
+            if (L > 0) {
+                int32_t pl;
+                bool ok;
+                if (ORIENTED) {
+                    // only the suffix of N+(a) ranked above the owner can hit
+                    pl = tasks[e].y;
+                    ok = pl > 0;
+                } else {
+                    const int32_t a = adj[vb.rb + (e - coff[b])];
+                    const VInfo va = vi[a];
+                    pl = va.len;
+                    ok = a != b && va.len >= 2 && vb.len >= 2;
+                }
                 if (ok) {
                     w = (uint32_t)est_items(min(L, P.lmax[2]), pl);
                     const int bin = owner_bin(L, P);
@@ -529,6 +603,7 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
                         if (bin == q) {
                             st_t[q] += 1;
                             st_b[q] += 4ull * (unsigned long long)(L + pl);
+                            st_i[q] += (unsigned long long)pl;
                         }
                 }
             }
@@ -551,9 +626,11 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
         for (int q = 0; q < 3; ++q) {
             unsigned long long t = wave_sum<unsigned long long>(st_t[q]);
             unsigned long long by = wave_sum<unsigned long long>(st_b[q]);
+            unsigned long long it = wave_sum<unsigned long long>(st_i[q]);
             if (lane == 0 && t) {
                 atomicAdd(&bstats[2 * q], t);
                 atomicAdd(&bstats[2 * q + 1], by);
+                atomicAdd(&bstats[6 + q], it);
             }
         }
     }
@@ -627,7 +704,8 @@ __global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restr
 struct CountArgs {
     const int32_t *adj;
     const VInfo *vi;
-    const int32_t *tpos;   // oriented: suffix start of each task entry
+    const int2 *tasks;     // oriented: (task vertex, suffix length) per task
+    const int64_t *toff;   // oriented: first task of each owner
     const int32_t *order;  // oriented: vertex of each rank (outputs are by vertex)
     uint32_t *out;
     unsigned long long *global;
@@ -689,6 +767,11 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
     __shared__ int32_t tpl[NW][kWave];
     __shared__ uint32_t tcnt[NW][kWave];
     __shared__ unsigned long long red[NW];
+    __shared__ int32_t wsum[NW];
+    // bitmap tiles (largest bin): words and their u16 prefixes share HB's space
+    constexpr bool kUseBitmap = LMAX >= 4096;
+    constexpr bool kChunkedLong = LMAX < 4096;
+    constexpr int kBmWords = (16 * NBMAX) / 6;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     const int nthr = NW * kWave;
@@ -698,7 +781,8 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
     for (int64_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const Job J = jobs[jb];
         const VInfo vo = A.vi[J.owner];
-        const int64_t obase = ORIENTED ? vo.rb + vo.dm : vo.rb;
+        const int64_t obase = vo.rb;
+        const int64_t tbase = ORIENTED ? A.toff[J.owner] : vo.rb;
         const int32_t Lfull = ORIENTED ? vo.len - vo.dm : vo.len;
         const int32_t s0 = J.tile * LMAX;
         const int32_t L = min(LMAX, Lfull - s0);
@@ -707,26 +791,70 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;         // buckets: 2^nbb >= L/2
         const BucketTable T{HB, nbb};
-        for (int i = threadIdx.x; i < (1 << nbb); i += nthr) HB[i] = make_int4(-1, -1, -1, -1);
-        if (ORIENTED)
-            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) C[i] = 0;
-        __syncthreads();
-        for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
-        __syncthreads();
         const int32_t lo_v = Sg[0], hi_v = Sg[L - 1];
+        // a tile whose value range is narrow enough gets a bitmap instead of
+        // the hash table (same LDS space)
+        const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
+        const int32_t nwords = (int32_t)((span + 31u) >> 5);
+        const bool use_bm = kUseBitmap && nwords <= kBmWords;
+        uint32_t *BMw = reinterpret_cast<uint32_t *>(HB);
+        uint16_t *PF = reinterpret_cast<uint16_t *>(BMw + kBmWords);
+        const TileBitmap BT{BMw, PF, lo_v, span};
+        if (use_bm) {
+            for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = 0u;
+            if (ORIENTED)
+                for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < L; i += nthr) {
+                const uint32_t off = (uint32_t)(Sg[i] - lo_v);
+                atomicOr(&BMw[off >> 5], 1u << (off & 31u));
+            }
+            __syncthreads();
+            // PF = exclusive prefix of the word popcounts (contiguous chunk per thread)
+            const int32_t per = (nwords + nthr - 1) / nthr;
+            const int32_t k0 = threadIdx.x * per, k1 = min(nwords, k0 + per);
+            int32_t mine = 0;
+            for (int32_t k = k0; k < k1; ++k) mine += __popc(BMw[k]);
+            const int32_t incl = wave_incl_scan_i32(mine);
+            if (lane == kWave - 1) wsum[wid] = incl;
+            __syncthreads();
+            int32_t run = incl - mine;
+            for (int w = 0; w < wid; ++w) run += wsum[w];
+            for (int32_t k = k0; k < k1; ++k) {
+                PF[k] = (uint16_t)run;
+                run += __popc(BMw[k]);
+            }
+            __syncthreads();
+        } else {
+            for (int i = threadIdx.x; i < (1 << nbb); i += nthr) HB[i] = make_int4(-1, -1, -1, -1);
+            if (ORIENTED)
+                for (int i = threadIdx.x; i < 4 << nbb; i += nthr) C[i] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
+            __syncthreads();
+        }
+        auto lookup = [&](int32_t w) -> int32_t { return use_bm ? BT.find(w) : T.find(w); };
         unsigned long long hits_job = 0;
 
         // tasks are dealt to the waves round-robin so short jobs still keep every wave busy
         for (int32_t g0 = 0; J.t_begin + (g0 * kWave) * NW + wid < J.t_end; ++g0) {
             const int32_t t = J.t_begin + (g0 * kWave + lane) * NW + wid;
             bool valid = t < J.t_end;
-            int32_t tv = valid ? A.adj[vo.rb + t] : 0;
-            const int32_t kp = (ORIENTED && valid) ? A.tpos[vo.rb + t] : 0;
+            int32_t tv = 0, sl = 0;
+            if (valid) {
+                if (ORIENTED) {
+                    const int2 tk = A.tasks[tbase + t];
+                    tv = tk.x;
+                    sl = tk.y;
+                } else {
+                    tv = A.adj[tbase + t];
+                }
+            }
             VInfo vt = {0, 0, 0};
             if (valid) vt = A.vi[tv];
             // oriented: the task list is the suffix of N+(tv) after the owner
-            int64_t pb = ORIENTED ? vt.rb + vt.dm + kp + 1 : vt.rb;
-            int32_t pl = ORIENTED ? vt.len - vt.dm - kp - 1 : vt.len;
+            int32_t pl = ORIENTED ? sl : vt.len;
+            int64_t pb = ORIENTED ? vt.rb + (vt.len - vt.dm) - sl : vt.rb;
             if (!ORIENTED && valid &&
                 (tv == J.owner || vt.len < 2 || vo.len < 2 || (A.skip_lower && tv < J.owner)))
                 valid = false;
@@ -755,34 +883,77 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             uint32_t mcount = 0;
             wave_sync();
 
-            // Long hash tasks run one at a time across the whole wave: the list
-            // base is wave-uniform (scalar registers), each lane streams every
-            // 64th entry through the table, and hits stay in registers.
-            uint64_t big = __ballot(strat == S_HASH && pl >= kBigTask);
-            while (big) {
-                const int jb2 = __ffsll((unsigned long long)big) - 1;
-                big &= big - 1;
-                const int32_t *lst = A.adj + readlane_i64(pb, jb2);
-                const int32_t n = __builtin_amdgcn_readlane(pl, jb2);
-                uint32_t h = 0;
-                for (int32_t k0 = lane; k0 < n; k0 += kBigUnroll * kWave) {
-                    int32_t w[kBigUnroll];
+            // Long hash tasks are streamed in 64-entry chunks across the whole
+            // wave: chunk c belongs to the task (lane) whose inclusive chunk
+            // count first exceeds c, so one batch of kBigUnroll loads per lane
+            // spans several tasks.  Task bases and lengths are wave-uniform
+            // (readlane), hit counts come from ballots.
+            const bool is_big = strat == S_HASH && pl >= kBigTask;
+            if (!kChunkedLong) {
+                // largest bin: one long task at a time across the wave
+                uint64_t big = __ballot(is_big);
+                while (big) {
+                    const int jb2 = __ffsll((unsigned long long)big) - 1;
+                    big &= big - 1;
+                    const int32_t *lst = A.adj + readlane_i64(pb, jb2);
+                    const int32_t n = __builtin_amdgcn_readlane(pl, jb2);
+                    uint32_t h = 0;
+                    for (int32_t k0 = lane; k0 < n; k0 += kBigUnroll * kWave) {
+                        int32_t w[kBigUnroll];
 #pragma unroll
-                    for (int u = 0; u < kBigUnroll; ++u) {
-                        const int32_t k = k0 + u * kWave;
-                        w[u] = (k < n) ? lst[k] : -1;
-                    }
+                        for (int u = 0; u < kBigUnroll; ++u) {
+                            const int32_t k = k0 + u * kWave;
+                            w[u] = (k < n) ? lst[k] : -1;
+                        }
 #pragma unroll
-                    for (int u = 0; u < kBigUnroll; ++u) {
-                        const int32_t slot = T.find(w[u]);
-                        if (slot >= 0) {
-                            ++h;
-                            if (ORIENTED) atomicAdd(&C[slot], 1u);
+                        for (int u = 0; u < kBigUnroll; ++u) {
+                            const int32_t slot = lookup(w[u]);
+                            if (slot >= 0) {
+                                ++h;
+                                if (ORIENTED) atomicAdd(&C[slot], 1u);
+                            }
                         }
                     }
+                    h = wave_sum<uint32_t>(h);
+                    if (lane == jb2) tcnt[wid][jb2] = h;
                 }
-                h = wave_sum<uint32_t>(h);
-                if (lane == jb2) tcnt[wid][jb2] = h;
+            } else if (__ballot(is_big)) {
+                const int32_t myc = is_big ? (pl + kWave - 1) / kWave : 0;
+                const int32_t cincl = wave_incl_scan_i32(myc);
+                const int32_t ctot = __builtin_amdgcn_readlane(cincl, kWave - 1);
+                int32_t cur_j = -1;
+                uint32_t cur_h = 0;
+                for (int32_t c0 = 0; c0 < ctot; c0 += kBigUnroll) {
+                    int32_t w[kBigUnroll], jj[kBigUnroll];
+#pragma unroll
+                    for (int u = 0; u < kBigUnroll; ++u) {
+                        const int32_t c = c0 + u;
+                        w[u] = -1;
+                        jj[u] = -1;
+                        if (c < ctot) {
+                            const int j = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
+                            const int32_t cfirst = __builtin_amdgcn_readlane(cincl - myc, j);
+                            const int32_t n = __builtin_amdgcn_readlane(pl, j);
+                            const int32_t *lst = A.adj + readlane_i64(pb, j);
+                            const int32_t k = (c - cfirst) * kWave + lane;
+                            if (k < n) w[u] = lst[k];
+                            jj[u] = j;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kBigUnroll; ++u) {
+                        const int32_t slot = lookup(w[u]);
+                        if (ORIENTED && slot >= 0) atomicAdd(&C[slot], 1u);
+                        const uint32_t hc = (uint32_t)__popcll(__ballot(slot >= 0));
+                        if (jj[u] != cur_j) {
+                            if (cur_j >= 0 && lane == 0) tcnt[wid][cur_j] = cur_h;
+                            cur_j = jj[u];
+                            cur_h = 0;
+                        }
+                        cur_h += hc;
+                    }
+                }
+                if (cur_j >= 0 && lane == 0) tcnt[wid][cur_j] = cur_h;
             }
 
             // Flattened phases over the remaining tasks of this wave: items of
@@ -830,7 +1001,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                     if (phase == 0) {
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
-                            const int32_t slot = T.find(ww[u]);
+                            const int32_t slot = lookup(ww[u]);
                             if (!ok[u]) continue;
                             if (jj[u] != myj) {
                                 if (myhits) atomicAdd(&tcnt[wid][myj], myhits);
@@ -879,7 +1050,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                             }
                             if (b0[u] < n0[u] && lst[u][b0[u]] == ww[u]) {
                                 ++myhits;
-                                if (ORIENTED) atomicAdd(&C[T.find(ww[u])], 1u);
+                                if (ORIENTED) atomicAdd(&C[lookup(ww[u])], 1u);
                             }
                         }
                     }
@@ -902,7 +1073,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                     int32_t x = lst[i], y = Sg[k];
                     if (x == y) {
                         ++mcount;
-                        if (ORIENTED) atomicAdd(&C[T.find(x)], 1u);
+                        if (ORIENTED) atomicAdd(&C[lookup(x)], 1u);
                     }
                     i += (x <= y);
                     k += (x >= y);
@@ -923,9 +1094,16 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         unsigned long long hjob = 0;
         for (int w = 0; w < NW; ++w) hjob += red[w];
         if (ORIENTED) {
-            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
-                const uint32_t ci = C[i];
-                if (ci) atomicAdd(&A.out[A.order[T.id_at(i)]], 2u * ci);
+            if (use_bm) {
+                for (int i = threadIdx.x; i < L; i += nthr) {
+                    const uint32_t ci = C[i];
+                    if (ci) atomicAdd(&A.out[A.order[Sg[i]]], 2u * ci);
+                }
+            } else {
+                for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
+                    const uint32_t ci = C[i];
+                    if (ci) atomicAdd(&A.out[A.order[T.id_at(i)]], 2u * ci);
+                }
             }
         }
         const int32_t owner_v = ORIENTED ? A.order[J.owner] : J.owner;
@@ -1072,69 +1250,87 @@ int prepare_views(Graph &g, bool oriented)
     TC_HIP(hipGetLastError());
     // all 32 bits: the cached maximum degree may predate batch updates
     TC_TRY(radix_sort_u64(keys, order, nv, 32, g.ov[4], g.ov[5], g.tmp, s));
-    int32_t *lr = reinterpret_cast<int32_t *>(keys);  // keys are no longer needed
-    hipLaunchKernelGGL(k_rank_of, dim3(vblocks), dim3(256), 0, s, order, g.st.len.as<int32_t>(), nv,
-                       rnk, lr);
+    int32_t *dpr = reinterpret_cast<int32_t *>(keys);  // keys are no longer needed
+    int32_t *dmr = dpr + nv;
+    hipLaunchKernelGGL(k_rank_of, dim3(vblocks), dim3(256), 0, s, order, g.st.len.as<int32_t>(), dmo,
+                       nv, rnk, dpr, dmr);
     TC_HIP(hipGetLastError());
 
-    // ---- rank layout: offsets, views, N+ parts
+    // ---- rank layout: plus array offsets (pcoff), task offsets (coff), views
+    TC_TRY(g.ov[6].ensure((size_t)(nv + 1) * sizeof(int64_t)));
+    int64_t *pcoff = g.ov[6].as<int64_t>();
+    TC_TRY(scan_counts_i32(dpr, pcoff, nv, g.tmp, s));
     TC_TRY(g.coff.ensure((size_t)(nv + 1) * sizeof(int64_t)));
-    int64_t *coff = g.coff.as<int64_t>();
-    TC_TRY(scan_counts_i32(lr, coff, nv, g.tmp, s));
+    int64_t *toff = g.coff.as<int64_t>();
+    TC_TRY(scan_counts_i32(dmr, toff, nv, g.tmp, s));
     TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
     VInfo *vi = g.vinfo.as<VInfo>();
-    hipLaunchKernelGGL(k_rank_vinfo, dim3(vblocks), dim3(256), 0, s, coff, order, dmo, lr, nv, vi);
-    TC_TRY(g.crow.ensure((size_t)(nchunks + 1) * sizeof(int32_t)));
-    int32_t *crow = g.crow.as<int32_t>();
-    hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, coff, nv, crow);
-    TC_TRY(g.padj.ensure((size_t)nnz * sizeof(int32_t)));
-    TC_TRY(g.tpos.ensure((size_t)nnz * sizeof(int32_t)));
+    hipLaunchKernelGGL(k_rank_vinfo, dim3(vblocks), dim3(256), 0, s, pcoff, dpr, dmr, nv, vi);
+    TC_HIP(hipGetLastError());
+    int64_t tot[2] = {0, 0};
+    TC_HIP(hipMemcpyAsync(&tot[0], pcoff + nv, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipMemcpyAsync(&tot[1], toff + nv, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    const int64_t nplus = tot[0], ntasks = tot[1];
+    TC_TRY(g.padj.ensure((size_t)std::max<int64_t>(nplus, 1) * sizeof(int32_t)));
+    TC_TRY(g.tasks.ensure((size_t)std::max<int64_t>(ntasks, 1) * sizeof(int2)));
     int32_t *padj = g.padj.as<int32_t>();
     hipLaunchKernelGGL(k_orient_write_plus, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
-                       g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, dmo, rnk, coff,
-                       nv, nnz, padj);
+                       g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, rnk, pcoff, nv,
+                       nnz, padj);
     TC_HIP(hipGetLastError());
 
     // ---- sort every N+ part by rank
-    const int64_t cap1 = std::max<int64_t>(1, nnz / 512 + 1);
-    const int64_t cap2 = std::max<int64_t>(1, nnz / 4096 + 1);
-    const int64_t cap3 = std::max<int64_t>(1, nnz / 16384 + 1);
-    TC_TRY(g.ov[6].ensure((size_t)(nv + cap1 + cap2 + cap3 + 4) * sizeof(int32_t)));
-    int32_t *nbig = g.ov[6].as<int32_t>();
-    int32_t *big0 = nbig + 4;
-    int32_t *big1 = big0 + nv;
-    int32_t *big2 = big1 + cap1;
-    int32_t *big3 = big2 + cap2;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(nbig),
-                       (int64_t)4, 0u);
-    hipLaunchKernelGGL(k_sort_plus_wave, dim3(grid_for(nv, 4, 1 << 16)), dim3(256), 0, s, vi, nv,
-                       padj, big0, big1, big2, big3, nbig);
+    const int64_t cap[5] = {std::max<int64_t>(1, std::min<int64_t>(nv, nplus / 9 + 1)),
+                            std::max<int64_t>(1, nplus / 65 + 1),
+                            std::max<int64_t>(1, nplus / 513 + 1),
+                            std::max<int64_t>(1, nplus / 4097 + 1),
+                            std::max<int64_t>(1, nplus / 16385 + 1)};
+    int64_t hoff[5];
+    int64_t lst_total = 0;
+    for (int q = 0; q < 5; ++q) {
+        hoff[q] = lst_total;
+        lst_total += cap[q];
+    }
+    TC_TRY(g.ov[7].ensure((size_t)lst_total * sizeof(int32_t)));
+    TC_TRY(g.ov[8].ensure(5 * sizeof(int64_t) + 8 * sizeof(int32_t)));
+    int32_t *lst = g.ov[7].as<int32_t>();
+    int64_t *loff = g.ov[8].as<int64_t>();
+    int32_t *lcnt = reinterpret_cast<int32_t *>(loff + 5);  // [5] counts, [5] symmetry flag
+    TC_HIP(hipMemcpyAsync(loff, hoff, sizeof(hoff), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(lcnt),
+                       (int64_t)8, 0u);
+    hipLaunchKernelGGL(k_sort_plus_small, dim3(vblocks), dim3(256), 0, s, vi, nv, padj, lst, loff,
+                       lcnt);
     TC_HIP(hipGetLastError());
     int dev = 0, cus = 0;
     TC_HIP(hipGetDevice(&dev));
     TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL((k_sort_plus_block<512, 64>), dim3(cus * 16), dim3(64), 0, s, vi, big0,
-                       nbig + 0, padj);
-    hipLaunchKernelGGL((k_sort_plus_block<4096, 256>), dim3(cus * 4), dim3(256), 0, s, vi, big1,
-                       nbig + 1, padj);
-    hipLaunchKernelGGL((k_sort_plus_block<16384, 1024>), dim3(cus), dim3(1024), 0, s, vi, big2,
-                       nbig + 2, padj);
+    hipLaunchKernelGGL(k_sort_plus_wave, dim3(cus * 8), dim3(256), 0, s, vi, lst + hoff[0], lcnt + 0,
+                       padj);
+    hipLaunchKernelGGL((k_sort_plus_block<512, 64>), dim3(cus * 16), dim3(64), 0, s, vi,
+                       lst + hoff[1], lcnt + 1, padj);
+    hipLaunchKernelGGL((k_sort_plus_block<4096, 256>), dim3(cus * 4), dim3(256), 0, s, vi,
+                       lst + hoff[2], lcnt + 2, padj);
+    hipLaunchKernelGGL((k_sort_plus_block<16384, 1024>), dim3(cus), dim3(1024), 0, s, vi,
+                       lst + hoff[3], lcnt + 3, padj);
     TC_HIP(hipGetLastError());
     int32_t nhuge = 0;
-    TC_HIP(hipMemcpyAsync(&nhuge, nbig + 3, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipMemcpyAsync(&nhuge, lcnt + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
     if (nhuge > 0) {
         // rows longer than any LDS tile: radix sort each one in global memory
         std::vector<int32_t> rows((size_t)nhuge);
-        TC_HIP(hipMemcpy(rows.data(), big3, (size_t)nhuge * sizeof(int32_t), hipMemcpyDeviceToHost));
+        TC_HIP(hipMemcpy(rows.data(), lst + hoff[4], (size_t)nhuge * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
         const int vbits = std::max(1, ceil_log2_i((int64_t)nv));
         for (int32_t r : rows) {
             VInfo x;
             TC_HIP(hipMemcpy(&x, vi + r, sizeof(VInfo), hipMemcpyDeviceToHost));
             const int64_t dp = x.len - x.dm;
-            TC_TRY(g.ov[7].ensure((size_t)dp * sizeof(uint64_t)));
-            uint64_t *k64 = g.ov[7].as<uint64_t>();
-            int32_t *P = padj + x.rb + x.dm;
+            TC_TRY(g.ov[9].ensure((size_t)dp * sizeof(uint64_t)));
+            uint64_t *k64 = g.ov[9].as<uint64_t>();
+            int32_t *P = padj + x.rb;
             hipLaunchKernelGGL(k_widen_u32, dim3(grid_for(dp, 256, 4096)), dim3(256), 0, s, P, dp, k64);
             TC_TRY(radix_sort_u64(k64, nullptr, dp, vbits, g.ov[4], g.ov[5], g.tmp, s));
             hipLaunchKernelGGL(k_narrow_u64, dim3(grid_for(dp, 256, 4096)), dim3(256), 0, s, k64, dp, P);
@@ -1142,15 +1338,20 @@ int prepare_views(Graph &g, bool oriented)
         }
     }
 
-    // ---- N- parts with the suffix starts
+    // ---- task array: (task vertex, suffix start) for every owner
+    const int64_t pchunks = ceil_div(nplus, kWave), tchunks = ceil_div(ntasks, kWave);
+    TC_TRY(g.ov[10].ensure((size_t)(pchunks + 1) * sizeof(int32_t)));
+    int32_t *pcrow = g.ov[10].as<int32_t>();
+    hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, pcoff, nv, pcrow);
+    TC_TRY(g.crow.ensure((size_t)(tchunks + 1) * sizeof(int32_t)));
+    hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, toff, nv, g.crow.as<int32_t>());
     int32_t *cursor = dmo;  // store-order |N-| is not needed any more
-    int32_t *bad = nbig + 3;
+    int32_t *bad = lcnt + 5;
     hipLaunchKernelGGL(k_fill_u32, dim3(vblocks), dim3(256), 0, s, reinterpret_cast<uint32_t *>(cursor),
                        (int64_t)nv, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(bad),
-                       (int64_t)1, 0u);
-    hipLaunchKernelGGL(k_orient_transpose, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, nv, nnz,
-                       padj, g.tpos.as<int32_t>(), cursor, bad);
+    if (nplus > 0)
+        hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 1 << 16)), dim3(256), 0, s,
+                           pcoff, pcrow, padj, vi, toff, nv, nplus, g.tasks.as<int2>(), cursor, bad);
     hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, nv, bad);
     TC_HIP(hipGetLastError());
     int32_t hbad = 0;
@@ -1161,6 +1362,7 @@ int prepare_views(Graph &g, bool oriented)
         set_last_error(TC_ERR_INVALID, "oriented layout: the rows are not symmetric");
         return TC_ERR_INVALID;
     }
+    g.ntasks = ntasks;
     g.oriented_views = true;
     return TC_OK;
 }
@@ -1232,7 +1434,10 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     int32_t *crow = g.crow.as<int32_t>();
     VInfo *vi = g.vinfo.as<VInfo>();
     const int32_t *adj = oriented ? g.padj.as<int32_t>() : g.st.pool.as<int32_t>();
-    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int2 *tasks = oriented ? g.tasks.as<int2>() : nullptr;
+    // planning walks the task array (oriented) or the store rows (direct)
+    const int64_t nent = oriented ? g.ntasks : nnz;
+    const int64_t nchunks = ceil_div(nent, kWave);
     const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
 
     // ---- planning: work per owner, jobs per bin
@@ -1252,10 +1457,10 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, work, (int64_t)nv);
     if (oriented)
         hipLaunchKernelGGL(k_owner_work<true>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj,
-                           g.tpos.as<int32_t>(), nv, nnz, P, work, stats ? acc + 4 : nullptr);
+                           tasks, nv, nent, P, work, stats ? acc + 4 : nullptr);
     else
         hipLaunchKernelGGL(k_owner_work<false>, dim3(wblocks), dim3(256), 0, s, coff, crow, vi, adj,
-                           nullptr, nv, nnz, P, work, stats ? acc + 4 : nullptr);
+                           nullptr, nv, nent, P, work, stats ? acc + 4 : nullptr);
     TC_HIP(hipGetLastError());
     TC_TRY(g.tmp3.ensure((size_t)3 * nv * sizeof(int32_t) + (size_t)3 * (nv + 1) * sizeof(int64_t) + 16));
     int32_t *njobs = g.tmp3.as<int32_t>();
@@ -1291,7 +1496,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     CountArgs A;
     A.adj = adj;
     A.vi = vi;
-    A.tpos = oriented ? g.tpos.as<int32_t>() : nullptr;
+    A.tasks = tasks;
+    A.toff = oriented ? coff : nullptr;
     A.order = oriented ? g.order.as<int32_t>() : nullptr;
     A.out = outp;
     A.global = acc;
@@ -1334,6 +1540,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
             st.jobs_bin[b] = jt[b];
             st.tasks_bin[b] = (int64_t)hacc[4 + 2 * b];
             st.bytes_alg_bin[b] = (double)hacc[5 + 2 * b];
+            st.items_bin[b] = (double)hacc[10 + b];
         }
         for (int q = 0; q < 3; ++q) st.strategy_tasks[q] = (int64_t)hacc[1 + q];
         st.triangles = oriented ? hacc[0] / 6 : 0;

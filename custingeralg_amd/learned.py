@@ -136,6 +136,57 @@ def autotune(g: cuStinger):
     return s, float(ms.value)
 
 
+def set_default_schedule(s: Schedule | None) -> None:
+    """Make s the process default schedule (None restores the built-in one)."""
+    check(lib().tc_schedule_set_default(C.byref(s) if s is not None else None))
+
+
+_SCHED_KEYS = ("small_max", "mid_max", "job_items", "ratio_x16", "persistent", "flush_items")
+
+
+def save_selector(path: str, model: CostModel, tuned: Schedule | None, meta: dict) -> None:
+    """Write the trained cost model and the tuned schedule as JSON."""
+    d = {"params": [float(x) for x in model.host_params()],
+         "schedule": ({k: int(getattr(tuned, k)) for k in _SCHED_KEYS} if tuned is not None
+                      else None),
+         "meta": meta}
+    with open(path, "w") as f:
+        json.dump(d, f, indent=1)
+
+
+def load_selector(path: str) -> dict:
+    """Load a saved selector: its BF16 strategy table becomes the learned
+    policy and its schedule the process default."""
+    from . import default_schedule
+    with open(path) as f:
+        d = json.load(f)
+    model = CostModel(np.array(d["params"], np.float32))
+    table = model.load()
+    sched = None
+    if d.get("schedule"):
+        set_default_schedule(None)
+        sched = default_schedule()
+        for k, v in d["schedule"].items():
+            setattr(sched, k, int(v))
+        set_default_schedule(sched)
+    return {"model": model, "table": table, "schedule": sched, "meta": d.get("meta", {})}
+
+
+def train_default(train_scales=(18, 20, 22), edge_factor: int = 16, samples_per_graph: int = 4096,
+                  steps: int = 400, seed: int = 0):
+    """Train the selector on R-MAT graphs of the given scales (generated on the
+    device); returns (model, table, losses, number of timed tasks)."""
+    from . import rmat_graph_device
+    graphs = [rmat_graph_device(s, edge_factor, 100 + s) for s in train_scales]
+    try:
+        model, table, losses, (feat, _) = train_selector(graphs, samples_per_graph, steps,
+                                                         seed=seed)
+    finally:
+        for g in graphs:
+            g.close()
+    return model, table, losses, len(feat)
+
+
 def compare_policies(g: cuStinger, reps: int = 3):
     """Best-of-reps time of the fixed heuristic and the learned table on one graph."""
     from . import default_schedule

@@ -1,0 +1,62 @@
+"""Train the learned strategy selector on the GPU and save it.
+
+    python -m custingeralg_amd.train_selector [--out PATH] [--tune-scale 24]
+
+Intersection timings come from R-MAT graphs of scales 18, 20 and 22 (edge
+factor 16), the cost model is trained by the library's own FP32 kernels, and
+the schedule is autotuned on an R-MAT graph of the tune scale (edge factor
+32, seed 5: a different graph from the benchmark's seed 4).  The JSON file
+holds the FP32 weights (rounded to BF16 when loaded), the tuned schedule and
+how both were made.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import device_name, rmat_graph_device
+from . import learned as Lm
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=Lm.default_model_path())
+    ap.add_argument("--scales", default="18,20,22")
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--tune-scale", type=int, default=24)
+    ap.add_argument("--tune-ef", type=int, default=32)
+    ap.add_argument("--tune-seed", type=int, default=5)
+    args = ap.parse_args(argv)
+    scales = tuple(int(x) for x in args.scales.split(","))
+    t0 = time.time()
+    model, table, losses, ntimed = Lm.train_default(scales, 16, args.samples, args.steps)
+    print(f"[train] {ntimed} timed tasks from R-MAT {scales}: loss {losses[0]:.4f} -> "
+          f"{losses[-1]:.4f} ({time.time() - t0:.1f}s)", flush=True)
+    t0 = time.time()
+    g = rmat_graph_device(args.tune_scale, args.tune_ef, args.tune_seed)
+    Lm.set_default_schedule(None)
+    tuned, ms = Lm.autotune(g)
+    g.close()
+    print(f"[train] autotune on R-MAT {args.tune_scale} ef {args.tune_ef} seed {args.tune_seed}: "
+          f"{ms:.1f} ms, small={tuned.small_max} mid={tuned.mid_max} flush={tuned.flush_items} "
+          f"job={tuned.job_items} ({time.time() - t0:.1f}s)", flush=True)
+    meta = {"device": device_name(), "train_scales": list(scales), "edge_factor": 16,
+            "timed_tasks": ntimed, "steps": args.steps, "loss_first": losses[0],
+            "loss_last": losses[-1],
+            "table_picks": np.bincount(table.ravel(), minlength=3).tolist(),
+            "tune_graph": f"R-MAT {args.tune_scale} ef {args.tune_ef} seed {args.tune_seed}",
+            "tune_ms": ms}
+    Lm.save_selector(args.out, model, tuned, meta)
+    Lm.unload()
+    Lm.set_default_schedule(None)
+    print(f"[train] wrote {os.path.relpath(args.out)}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

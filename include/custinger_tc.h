@@ -135,7 +135,8 @@ typedef struct {
     int64_t strategy_tasks[3]; /* tasks run as merge / bsearch / hash         */
     uint64_t triangles;     /* T for simple graphs (global/6), else 0         */
     int32_t method_used;
-    int32_t reserved[7];
+    int32_t reserved0;
+    double items_bin[3];    /* task-list entries the tasks of each bin stream */
 } tc_stats;
 
 /* ------------------------------------------------------ static count -- */
@@ -280,6 +281,10 @@ int tc_model_table(uint8_t *table_host, int32_t *dim);
 /* Sweeps bin thresholds and job grain on g and stores the best schedule as
  * the process default (the reference's exhaustive sweep, tests/cct.cpp:177-209). */
 int tc_autotune(tc_graph *g, tc_schedule *best, double *best_ms);
+/* Makes s the process default schedule (what tc_schedule_default returns and
+ * what a NULL schedule means), e.g. one saved from an earlier tc_autotune;
+ * NULL restores the built-in default.  No counterpart in the reference. */
+int tc_schedule_set_default(const tc_schedule *s);
 
 #ifdef __cplusplus
 }
