@@ -68,7 +68,7 @@ class Stats(C.Structure):
                 ("tasks_bin", C.c_int64 * 4), ("jobs_bin", C.c_int64 * 4),
                 ("strategy_tasks", C.c_int64 * 3), ("triangles", C.c_uint64),
                 ("method_used", C.c_int32), ("reserved0", C.c_int32),
-                ("items_bin", C.c_double * 3)]
+                ("items_bin", C.c_double * 3), ("stream_bytes_bin", C.c_double * 3)]
 
     def as_dict(self) -> dict:
         return {
@@ -76,6 +76,7 @@ class Stats(C.Structure):
             "ms_bin": list(self.ms_bin)[:3], "bytes_alg_bin": list(self.bytes_alg_bin)[:3],
             "tasks_bin": list(self.tasks_bin)[:3], "jobs_bin": list(self.jobs_bin)[:3],
             "items_bin": list(self.items_bin),
+            "stream_bytes_bin": list(self.stream_bytes_bin),
             "strategy_tasks": dict(zip(("merge", "bsearch", "hash"), self.strategy_tasks)),
             "triangles": int(self.triangles), "method_used": int(self.method_used),
         }

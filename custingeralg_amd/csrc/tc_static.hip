@@ -636,12 +636,20 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
     }
 }
 
+// Jobs per owner and bin; jstats[b] (optional) gets the owner-tile entries
+// the jobs of bin b load.
 template <bool ORIENTED>
 __global__ void k_plan_jobs(const VInfo *__restrict__ vi, const unsigned long long *__restrict__ work,
-                            int32_t nv, PlanParams P, int32_t *__restrict__ njobs /* [3][nv] */)
+                            int32_t nv, PlanParams P, int32_t *__restrict__ njobs /* [3][nv] */,
+                            unsigned long long *__restrict__ jstats)
 {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long tile_items[3] = {0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); v0 < nv;
+         v0 += stride) {
+        const int64_t v = v0 + lane;
+        if (v >= nv) continue;
         VInfo x = vi[v];
         int32_t L = ORIENTED ? x.len - x.dm : x.len;
         int32_t T = ORIENTED ? x.dm : x.len;
@@ -660,10 +668,20 @@ __global__ void k_plan_jobs(const VInfo *__restrict__ vi, const unsigned long lo
             if (split > T) split = T;
             if (split < 1) split = 1;
             nj[bin] = (int32_t)(split * tiles);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q == bin) tile_items[q] += (unsigned long long)split * (unsigned long long)L;
         }
         njobs[v] = nj[0];
         njobs[(int64_t)nv + v] = nj[1];
         njobs[2 * (int64_t)nv + v] = nj[2];
+    }
+    if (jstats) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const unsigned long long t = wave_sum<unsigned long long>(tile_items[q]);
+            if (lane == 0 && t) atomicAdd(&jstats[q], t);
+        }
     }
 }
 
@@ -1467,10 +1485,10 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     int64_t *jscan = reinterpret_cast<int64_t *>(njobs + 3 * (int64_t)nv + ((3 * (int64_t)nv) & 1));
     if (oriented)
         hipLaunchKernelGGL(k_plan_jobs<true>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
-                           work, nv, P, njobs);
+                           work, nv, P, njobs, stats ? acc + 13 : nullptr);
     else
         hipLaunchKernelGGL(k_plan_jobs<false>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
-                           work, nv, P, njobs);
+                           work, nv, P, njobs, stats ? acc + 13 : nullptr);
     TC_HIP(hipGetLastError());
     for (int b = 0; b < 3; ++b)
         TC_TRY(scan_counts_i32(njobs + (int64_t)b * nv, jscan + (int64_t)b * (nv + 1), nv, g.tmp, s));
@@ -1541,6 +1559,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
             st.tasks_bin[b] = (int64_t)hacc[4 + 2 * b];
             st.bytes_alg_bin[b] = (double)hacc[5 + 2 * b];
             st.items_bin[b] = (double)hacc[10 + b];
+            st.stream_bytes_bin[b] = 4.0 * (double)hacc[10 + b] + 24.0 * (double)hacc[4 + 2 * b] +
+                                     4.0 * (double)hacc[13 + b];
         }
         for (int q = 0; q < 3; ++q) st.strategy_tasks[q] = (int64_t)hacc[1 + q];
         st.triangles = oriented ? hacc[0] / 6 : 0;

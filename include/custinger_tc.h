@@ -137,6 +137,9 @@ typedef struct {
     int32_t method_used;
     int32_t reserved0;
     double items_bin[3];    /* task-list entries the tasks of each bin stream */
+    double stream_bytes_bin[3]; /* bytes each bin must read: 4 per streamed
+                                   entry + 24 per task (task entry and vertex
+                                   record) + 4 per owner-tile entry per job */
 } tc_stats;
 
 /* ------------------------------------------------------ static count -- */
