@@ -429,6 +429,7 @@ void graph_destroy(Graph *g)
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : g->sb) b.release();
     for (DevBuf &b : g->ov) b.release();
+    for (DevBuf &b : g->lbuf) b.release();
     if (g->events_ready)
         for (auto &e : g->ev) (void)hipEventDestroy(e);
     delete g;

@@ -84,6 +84,15 @@ struct VInfo {
     int32_t len;
 };
 
+// One task of the oriented layout: task vertex a (rank) of an owner b, and the
+// suffix of N+(a) ranked above b -- the only entries that can close a
+// triangle -- as its start in the plus array and its length.
+struct TaskRec {
+    int32_t a;
+    int32_t len;
+    int64_t base;
+};
+
 // Intersection jobs: a slice [t_begin, t_end) of an owner's task list and the
 // tile of the owner's list that sits in LDS.
 struct Job {
@@ -126,6 +135,7 @@ struct Graph {
     // per-vertex marks, and batch scratch
     DevBuf flagpool, vmark;
     DevBuf sb[24];
+    DevBuf lbuf[3];  // load-balanced delta: items, their prefix, per-edge counts
     bool degree_stale = false;
     // batched counting: graph boundaries and per-graph output (device)
     const int64_t *batch_voff = nullptr;

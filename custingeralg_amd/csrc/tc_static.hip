@@ -492,7 +492,7 @@ __global__ void k_narrow_u64(const uint64_t *__restrict__ y, int64_t n, int32_t 
 __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int32_t *__restrict__ pcrow,
                                    const int32_t *__restrict__ padj, const VInfo *__restrict__ vi,
                                    const int64_t *__restrict__ toff, int32_t nv, int64_t nplus,
-                                   int2 *__restrict__ tasks, int32_t *__restrict__ cursor,
+                                   TaskRec *__restrict__ tasks, int32_t *__restrict__ cursor,
                                    int32_t *__restrict__ bad)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -506,7 +506,7 @@ __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int3
         const int32_t w = padj[e];
         const int32_t slot = atomicAdd(&cursor[w], 1);
         if (slot < vi[w].dm)
-            tasks[toff[w] + slot] = make_int2(r, (int32_t)(pcoff[r + 1] - e - 1));
+            tasks[toff[w] + slot] = TaskRec{r, (int32_t)(pcoff[r + 1] - e - 1), e + 1};
         else
             atomicOr(bad, 1);  // rows are not symmetric
     }
@@ -564,7 +564,7 @@ template <bool ORIENTED>
 // their row offsets coff and chunk rows crow.
 __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
                              const VInfo *__restrict__ vi,
-                             const int32_t *__restrict__ adj, const int2 *__restrict__ tasks,
+                             const int32_t *__restrict__ adj, const TaskRec *__restrict__ tasks,
                              int32_t nv, int64_t nnz, PlanParams P,
                              unsigned long long *__restrict__ work,
                              unsigned long long *__restrict__ bstats)
@@ -587,7 +587,7 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
                 bool ok;
                 if (ORIENTED) {
                     // only the suffix of N+(a) ranked above the owner can hit
-                    pl = tasks[e].y;
+                    pl = tasks[e].len;
                     ok = pl > 0;
                 } else {
                     const int32_t a = adj[vb.rb + (e - coff[b])];
@@ -722,7 +722,7 @@ __global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restr
 struct CountArgs {
     const int32_t *adj;
     const VInfo *vi;
-    const int2 *tasks;     // oriented: (task vertex, suffix length) per task
+    const TaskRec *tasks;  // oriented: task vertex and suffix of each task
     const int64_t *toff;   // oriented: first task of each owner
     const int32_t *order;  // oriented: vertex of each rank (outputs are by vertex)
     uint32_t *out;
@@ -859,20 +859,22 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             const int32_t t = J.t_begin + (g0 * kWave + lane) * NW + wid;
             bool valid = t < J.t_end;
             int32_t tv = 0, sl = 0;
+            int64_t sb = 0;
             if (valid) {
                 if (ORIENTED) {
-                    const int2 tk = A.tasks[tbase + t];
-                    tv = tk.x;
-                    sl = tk.y;
+                    const TaskRec tk = A.tasks[tbase + t];
+                    tv = tk.a;
+                    sl = tk.len;
+                    sb = tk.base;
                 } else {
                     tv = A.adj[tbase + t];
                 }
             }
             VInfo vt = {0, 0, 0};
-            if (valid) vt = A.vi[tv];
+            if (!ORIENTED && valid) vt = A.vi[tv];
             // oriented: the task list is the suffix of N+(tv) after the owner
             int32_t pl = ORIENTED ? sl : vt.len;
-            int64_t pb = ORIENTED ? vt.rb + (vt.len - vt.dm) - sl : vt.rb;
+            int64_t pb = ORIENTED ? sb : vt.rb;
             if (!ORIENTED && valid &&
                 (tv == J.owner || vt.len < 2 || vo.len < 2 || (A.skip_lower && tv < J.owner)))
                 valid = false;
@@ -1291,7 +1293,7 @@ int prepare_views(Graph &g, bool oriented)
     TC_HIP(hipStreamSynchronize(s));
     const int64_t nplus = tot[0], ntasks = tot[1];
     TC_TRY(g.padj.ensure((size_t)std::max<int64_t>(nplus, 1) * sizeof(int32_t)));
-    TC_TRY(g.tasks.ensure((size_t)std::max<int64_t>(ntasks, 1) * sizeof(int2)));
+    TC_TRY(g.tasks.ensure((size_t)std::max<int64_t>(ntasks, 1) * sizeof(TaskRec)));
     int32_t *padj = g.padj.as<int32_t>();
     hipLaunchKernelGGL(k_orient_write_plus, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
                        g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, rnk, pcoff, nv,
@@ -1369,7 +1371,8 @@ int prepare_views(Graph &g, bool oriented)
                        (int64_t)nv, 0u);
     if (nplus > 0)
         hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 1 << 16)), dim3(256), 0, s,
-                           pcoff, pcrow, padj, vi, toff, nv, nplus, g.tasks.as<int2>(), cursor, bad);
+                           pcoff, pcrow, padj, vi, toff, nv, nplus, g.tasks.as<TaskRec>(), cursor,
+                           bad);
     hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, nv, bad);
     TC_HIP(hipGetLastError());
     int32_t hbad = 0;
@@ -1452,7 +1455,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     int32_t *crow = g.crow.as<int32_t>();
     VInfo *vi = g.vinfo.as<VInfo>();
     const int32_t *adj = oriented ? g.padj.as<int32_t>() : g.st.pool.as<int32_t>();
-    const int2 *tasks = oriented ? g.tasks.as<int2>() : nullptr;
+    const TaskRec *tasks = oriented ? g.tasks.as<TaskRec>() : nullptr;
     // planning walks the task array (oriented) or the store rows (direct)
     const int64_t nent = oriented ? g.ntasks : nnz;
     const int64_t nchunks = ceil_div(nent, kWave);

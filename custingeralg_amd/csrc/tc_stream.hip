@@ -26,16 +26,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ inline int32_t wave_scan(int32_t x, int lane)
-{
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        int32_t y = __shfl_up(x, d, kWave);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-
 __device__ inline int64_t find_pos(const int32_t *__restrict__ pool, int64_t base, int32_t n,
                                    int32_t key)
 {
@@ -368,94 +358,108 @@ __global__ void k_clear_flags(const int64_t *__restrict__ pos, int64_t n, uint8_
         flag[pos[i]] = 0;
 }
 
-// Delta over undirected batch pairs present in G* (sel[i] != 0): the shorter
+// Delta over undirected batch pairs present in G* (sel[t] != 0): the shorter
 // list of s and d is searched entry by entry in the longer one; each common
-// neighbour w adds sign*(2 - bs - bd) to w and sign*(1 + [bs=bd=0]) to s and d.
-// Tasks are flattened across the 64 lanes; hits stay in registers per task.
-__global__ __launch_bounds__(kThreads) void k_delta(
-    const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
-    const int32_t *__restrict__ sel, int64_t n, const int32_t *__restrict__ pool,
-    const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
-    const uint8_t *__restrict__ flag, int32_t sign, uint32_t *__restrict__ out,
-    unsigned long long *__restrict__ gsum)
+// neighbour w adds sign*(2 - bs - bd) to w and sign*(1 + [bs=bd=0]) to s and d,
+// where bs / bd flag (s,w) / (d,w) as batch edges.  The items (entries of the
+// shorter list of every selected edge) are numbered by an exclusive prefix
+// pre[] over the edges and dealt to the waves in 64-item chunks, so an edge
+// between two hubs is spread over many waves instead of one.
+__global__ void k_delta_items(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+                              const int32_t *__restrict__ sel, int64_t n,
+                              const int32_t *__restrict__ len, int32_t *__restrict__ items)
 {
-    constexpr int NW = kThreads / kWave;
-    __shared__ int32_t pre[NW][kWave + 1];
-    __shared__ int64_t tsb[NW][kWave], tlb[NW][kWave];
-    __shared__ int32_t tll[NW][kWave];
-    __shared__ uint8_t tss[NW][kWave];
-    __shared__ uint32_t tcnt[NW][kWave];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        items[t] = (sel == nullptr || sel[t]) ? min(len[pa[t]], len[pb[t]]) : 0;
+}
+
+// Largest t in [lo, hi] with pre[t] <= i.
+__device__ inline int64_t edge_of(const int64_t *__restrict__ pre, int64_t i, int64_t lo, int64_t hi)
+{
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo + 1) >> 1);
+        if (pre[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_delta_lb(
+    const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+    const int64_t *__restrict__ pre, int64_t n, int64_t total,
+    const int32_t *__restrict__ pool, const int64_t *__restrict__ boff,
+    const int32_t *__restrict__ len, const uint8_t *__restrict__ flag, int32_t sign,
+    uint32_t *__restrict__ out, uint32_t *__restrict__ ecnt, unsigned long long *__restrict__ gsum)
+{
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
-    const int64_t nwaves = (int64_t)gridDim.x * NW;
+    const int64_t nchunks = (total + kWave - 1) / kWave;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     int64_t local = 0;
-    for (int64_t g0 = ((int64_t)blockIdx.x * NW + wid) * kWave; g0 < n; g0 += nwaves * kWave) {
-        const int64_t t = g0 + lane;
-        bool valid = t < n && (sel == nullptr || sel[t]);
-        int32_t s = 0, d = 0, ls = 0, ld = 0;
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        const int64_t i0 = c * kWave;
+        const int64_t ilast = min(i0 + kWave, total) - 1;
+        int64_t tb = 0;
+        if (lane == 0) tb = edge_of(pre, i0, 0, n - 1);
+        if (lane == 1) tb = edge_of(pre, ilast, 0, n - 1);
+        const int64_t tfirst = __shfl(tb, 0, kWave), tlast = __shfl(tb, 1, kWave);
+        const int64_t i = i0 + lane;
+        const bool valid = i <= ilast;
+        const int64_t t = valid ? edge_of(pre, i, tfirst, tlast) : tlast;
+        uint32_t m = 0;
         if (valid) {
-            s = pa[t];
-            d = pb[t];
-            ls = len[s];
-            ld = len[d];
-        }
-        const bool s_short = ls <= ld;
-        tsb[wid][lane] = valid ? (s_short ? boff[s] : boff[d]) : 0;
-        tlb[wid][lane] = valid ? (s_short ? boff[d] : boff[s]) : 0;
-        tll[wid][lane] = s_short ? ld : ls;
-        tss[wid][lane] = s_short ? 1 : 0;
-        tcnt[wid][lane] = 0;
-        const int32_t items = valid ? min(ls, ld) : 0;
-        const int32_t incl = wave_scan(items, lane);
-        pre[wid][lane + 1] = incl;
-        if (lane == 0) pre[wid][0] = 0;
-        const int32_t total = __shfl(incl, kWave - 1, kWave);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        int32_t j = 0, myj = -1;
-        uint32_t mine = 0;
-        for (int32_t it = lane; it < total; it += kWave) {
-            if (pre[wid][j + 1] <= it) {
-                int32_t lo = j + 1, hi = kWave;
-                while (hi - lo > 1) {
-                    int32_t mid = (lo + hi) >> 1;
-                    if (pre[wid][mid] <= it) lo = mid; else hi = mid;
-                }
-                j = lo;
-            }
-            if (j != myj) {
-                if (mine) atomicAdd(&tcnt[wid][myj], mine);
-                myj = j;
-                mine = 0;
-            }
-            const int32_t k = it - pre[wid][j];
-            const int64_t sb = tsb[wid][j], lb = tlb[wid][j];
+            const int32_t s = pa[t], d = pb[t];
+            const int32_t ls = len[s], ld = len[d];
+            const bool s_short = ls <= ld;
+            const int64_t sb = s_short ? boff[s] : boff[d];
+            const int64_t lb = s_short ? boff[d] : boff[s];
+            const int32_t ll = s_short ? ld : ls;
+            const int64_t k = i - pre[t];
             const int32_t x = pool[sb + k];
-            const int64_t p = find_pos(pool, lb, tll[wid][j], x);
+            const int64_t p = find_pos(pool, lb, ll, x);
             if (p >= 0) {
                 const int fs = flag[sb + k], fl = flag[p];
-                const int bs = tss[wid][j] ? fs : fl;
-                const int bd = tss[wid][j] ? fl : fs;
+                const int bs = s_short ? fs : fl;
+                const int bd = s_short ? fl : fs;
                 const int cw = 2 - bs - bd;
                 if (cw) atomicAdd(&out[x], (uint32_t)(sign * cw));
                 local += sign * cw;
-                mine += (bs | bd) ? 1u : 2u;
+                m = (bs | bd) ? 1u : 2u;
             }
         }
-        if (mine) atomicAdd(&tcnt[wid][myj], mine);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t c = tcnt[wid][lane];
-        if (valid && c) {
-            atomicAdd(&out[s], (uint32_t)(sign * (int32_t)c));
-            atomicAdd(&out[d], (uint32_t)(sign * (int32_t)c));
-            local += 2 * (int64_t)sign * c;
+        // segmented sum of m over lanes of the same edge (t is non-decreasing)
+        uint32_t x = m;
+#pragma unroll
+        for (int dd = 1; dd < kWave; dd <<= 1) {
+            const uint32_t y = __shfl_up(x, dd, kWave);
+            const int64_t ty = __shfl_up(t, dd, kWave);
+            if (lane >= dd && ty == t) x += y;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        const int64_t tn = __shfl_down(t, 1, kWave);
+        const bool tail = lane == kWave - 1 || tn != t || i + 1 > ilast;
+        if (valid && tail && x) atomicAdd(&ecnt[t], x);
     }
     for (int dd = 32; dd > 0; dd >>= 1) local += __shfl_xor(local, dd);
     if (lane == 0 && local) atomicAdd(gsum, (unsigned long long)local);
+}
+
+// Per edge: both endpoints get the edge's count.
+__global__ void k_delta_edges(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+                              int64_t n, const uint32_t *__restrict__ ecnt, int32_t sign,
+                              uint32_t *__restrict__ out, unsigned long long *__restrict__ gsum)
+{
+    int64_t local = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = ecnt[t];
+        if (c) {
+            atomicAdd(&out[pa[t]], (uint32_t)(sign * (int32_t)c));
+            atomicAdd(&out[pb[t]], (uint32_t)(sign * (int32_t)c));
+            local += 2 * (int64_t)sign * c;
+        }
+    }
+    for (int dd = 32; dd > 0; dd >>= 1) local += __shfl_xor(local, dd);
+    if ((threadIdx.x & (kWave - 1)) == 0 && local) atomicAdd(gsum, (unsigned long long)local);
 }
 
 // The reference's three passes at one lane per directed batch entry
@@ -606,6 +610,48 @@ template <class T> T *buf(Graph &g, int i, int64_t n)
         if (!(p)) return TC_ERR_OOM;                                                   \
     } while (0)
 
+__global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+int64_t read_i64(const int64_t *d, hipStream_t s);
+
+// Delta of the selected edges (pa[t], pb[t]) with the given sign, load
+// balanced over the items of the whole batch.
+int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel, int64_t n,
+              int32_t sign, uint32_t *out, unsigned long long *gsum)
+{
+    if (n <= 0) return TC_OK;
+    hipStream_t s = g.stream;
+    TC_TRY(g.lbuf[0].ensure((size_t)n * sizeof(int32_t)));
+    TC_TRY(g.lbuf[1].ensure((size_t)(n + 1) * sizeof(int64_t)));
+    TC_TRY(g.lbuf[2].ensure((size_t)n * sizeof(uint32_t)));
+    int32_t *items = g.lbuf[0].as<int32_t>();
+    int64_t *pre = g.lbuf[1].as<int64_t>();
+    uint32_t *ecnt = g.lbuf[2].as<uint32_t>();
+    hipLaunchKernelGGL(k_delta_items, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, sel, n,
+                       g.st.len.as<int32_t>(), items);
+    TC_TRY(scan_counts_i32(items, pre, n, g.tmp, s));
+    const int64_t total = read_i64(pre + n, s);
+    if (total < 0) {
+        set_last_error(TC_ERR_HIP, "delta: item count read-back failed");
+        return TC_ERR_HIP;
+    }
+    if (total == 0) return TC_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, ecnt, n);
+    const int64_t nchunks = (total + kWave - 1) / kWave;
+    hipLaunchKernelGGL(k_delta_lb, dim3(grid_for(nchunks, kThreads / kWave, 1 << 16)), dim3(kThreads),
+                       0, s, pa, pb, pre, n, total, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                       g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), sign, out, ecnt, gsum);
+    hipLaunchKernelGGL(k_delta_edges, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, n, ecnt,
+                       sign, out, gsum);
+    TC_HIP(hipGetLastError());
+    return TC_OK;
+}
+
 int64_t read_i64(const int64_t *d, hipStream_t s)
 {
     int64_t h = 0;
@@ -703,9 +749,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
                            pres, nD, (uint8_t)1, g.flagpool.as<uint8_t>());
         if (count) {
             TC_HIP(hipEventRecord(g.ev[9], s));
-            hipLaunchKernelGGL(k_delta, dim3(grid_for(nD, kThreads, 1 << 16)), dim3(kThreads), 0, s,
-                               da, db, pres, nD, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
-                               g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), -1, out, acc + 2);
+            TC_TRY(run_delta(g, da, db, pres, nD, -1, out, acc + 2));
             TC_HIP(hipEventRecord(g.ev[10], s));
         }
         hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc + 4, 1);
@@ -803,9 +847,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
             g.st.nnz += nd;
             if (count) {
                 TC_HIP(hipEventRecord(g.ev[11], s));
-                hipLaunchKernelGGL(k_delta, dim3(grid_for(nI, kThreads, 1 << 16)), dim3(kThreads), 0, s,
-                                   ia, ib, absent, nI, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
-                                   g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), 1, out, acc + 2);
+                TC_TRY(run_delta(g, ia, ib, absent, nI, 1, out, acc + 2));
                 TC_HIP(hipEventRecord(g.ev[12], s));
             }
             hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, newpos, nd,
