@@ -46,6 +46,7 @@ EXPORTS = [
     "tc_batched_count", "tc_lb_partition", "tc_lb_degree_bins", "tc_gen_rmat_pairs",
     "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune", "tc_schedule_set_default",
+    "tc_batched_graph_create", "tc_batched_graph_count",
 ]
 
 
@@ -149,6 +150,8 @@ def lib():
         L.tc_model_table.argtypes = [vp, C.POINTER(i32)]
         L.tc_autotune.argtypes = [vp, C.POINTER(Schedule), C.POINTER(C.c_double)]
         L.tc_schedule_set_default.argtypes = [C.POINTER(Schedule)]
+        L.tc_batched_graph_create.argtypes = [i32, vp, vp, vp, i64, i32, vp, C.POINTER(vp)]
+        L.tc_batched_graph_count.argtypes = [vp, vp, vp, C.POINTER(Schedule), C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -389,6 +392,40 @@ def batched_count_device(voff: DeviceArray, off: DeviceArray, adj: DeviceArray,
     check(lib().tc_batched_count(voff.n - 1, voff.ptr, off.ptr, adj.ptr, adj.n, off.n - 1,
                                  per_graph.ptr, None, sp, C.byref(st), None))
     return st.as_dict()
+
+
+class BatchedGraph:
+    """G graphs concatenated with global vertex ids (device arrays), validated
+    once and counted repeatedly; the arrays must outlive this object."""
+
+    def __init__(self, voff: DeviceArray, off: DeviceArray, adj: DeviceArray):
+        self._keep = (voff, off, adj)
+        self.G = voff.n - 1
+        self.nv = off.n - 1
+        h = C.c_void_p()
+        check(lib().tc_batched_graph_create(self.G, voff.ptr, off.ptr, adj.ptr, adj.n, self.nv,
+                                            None, C.byref(h)))
+        self.h = h.value
+
+    def count(self, per_graph: DeviceArray, per_vertex: DeviceArray | None = None,
+              schedule: Schedule | None = None) -> dict:
+        st = Stats()
+        sp = C.byref(schedule) if schedule is not None else None
+        check(lib().tc_batched_graph_count(self.h, per_graph.ptr,
+                                           per_vertex.ptr if per_vertex is not None else None, sp,
+                                           C.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self.h:
+            lib().tc_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------ reference shape

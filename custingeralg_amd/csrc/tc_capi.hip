@@ -40,6 +40,10 @@ int mlp_forward(const float *params, const float *feat, int64_t n, float *pred);
 int model_load(const float *params_host);
 int model_table(uint8_t *table, int32_t *dim);
 int autotune(Graph &g, tc_schedule *best, double *best_ms);
+int batched_graph_create(int32_t G, const int64_t *voff, const int64_t *off, const int32_t *adj,
+                         int64_t nnz, int32_t nv_total, hipStream_t s, Graph **out);
+int batched_graph_count(Graph &g, uint64_t *per_graph, uint32_t *per_vertex,
+                        const tc_schedule &sched, tc_stats *stats);
 int set_default_schedule(const tc_schedule *s);
 bool schedule_override(tc_schedule *s);
 }  // namespace tc
@@ -324,6 +328,35 @@ int tc_batched_count(int32_t G, const int64_t *voff_dev, const int64_t *off_dev,
         if (sched) s = *sched; else tc_schedule_default(&s);
         return tc::batched_count(G, voff_dev, off_dev, adj_dev, nnz, nv_total, per_graph_dev,
                                  per_vertex_dev, s, stats, (hipStream_t)hip_stream);
+    })
+}
+
+int tc_batched_graph_create(int32_t G, const int64_t *voff_dev, const int64_t *off_dev,
+                            const int32_t *adj_dev, int64_t nnz, int32_t nv_total, void *hip_stream,
+                            tc_graph **out)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(out && G > 0 && nv_total >= 0 && nnz >= 0 && voff_dev && off_dev &&
+                         (nnz == 0 || adj_dev),
+                     "tc_batched_graph_create: bad arguments");
+        *out = nullptr;
+        tc::Graph *g = nullptr;
+        TC_TRY(tc::batched_graph_create(G, voff_dev, off_dev, adj_dev, nnz, nv_total,
+                                        (hipStream_t)hip_stream, &g));
+        *out = new tc_graph{g};
+        return TC_OK;
+    })
+}
+
+int tc_batched_graph_count(tc_graph *g, uint64_t *per_graph_dev, uint32_t *per_vertex_dev,
+                           const tc_schedule *sched, tc_stats *stats)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(g && per_graph_dev, "tc_batched_graph_count: bad arguments");
+        TC_TRY(check_schedule(sched));
+        tc_schedule s;
+        if (sched) s = *sched; else tc_schedule_default(&s);
+        return tc::batched_graph_count(*g->g, per_graph_dev, per_vertex_dev, s, stats);
     })
 }
 

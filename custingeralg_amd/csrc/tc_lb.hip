@@ -115,21 +115,48 @@ int lb_degree_bins(Graph &g, const int32_t *verts, int64_t len, int64_t counts[3
     return TC_OK;
 }
 
+int batched_graph_create(int32_t G, const int64_t *voff, const int64_t *off, const int32_t *adj,
+                         int64_t nnz, int32_t nv_total, hipStream_t s, Graph **out)
+{
+    *out = nullptr;
+    Graph *g = nullptr;
+    TC_TRY(graph_borrow_csr(nv_total, nnz, off, adj, s, &g));
+    g->batch_voff = voff;
+    g->batch_ngraphs = G;
+    const int st = validate_graph(*g, true, nullptr);
+    if (st != TC_OK) {
+        graph_destroy(g);
+        return st;
+    }
+    *out = g;
+    return TC_OK;
+}
+
+int batched_graph_count(Graph &g, uint64_t *per_graph, uint32_t *per_vertex,
+                        const tc_schedule &sched, tc_stats *stats)
+{
+    if (g.batch_ngraphs <= 0 || !g.batch_voff) {
+        set_last_error(TC_ERR_INVALID, "tc_batched_graph_count: not a batched graph handle");
+        return TC_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_zero_ull, dim3(grid_for(g.batch_ngraphs, 256, 4096)), dim3(256), 0,
+                       g.stream, reinterpret_cast<unsigned long long *>(per_graph),
+                       (int64_t)g.batch_ngraphs);
+    g.batch_per_graph = reinterpret_cast<unsigned long long *>(per_graph);
+    uint64_t glob = 0;
+    const int st = static_count(g, per_vertex, &glob, sched, stats, false);
+    g.batch_per_graph = nullptr;
+    return st;
+}
+
 int batched_count(int32_t G, const int64_t *voff, const int64_t *off, const int32_t *adj,
                   int64_t nnz, int32_t nv_total, uint64_t *per_graph, uint32_t *per_vertex,
                   const tc_schedule &sched, tc_stats *stats, hipStream_t s)
 {
     if (G == 0) return TC_OK;
     Graph *g = nullptr;
-    TC_TRY(graph_borrow_csr(nv_total, nnz, off, adj, s, &g));
-    hipLaunchKernelGGL(k_zero_ull, dim3(grid_for(G, 256, 4096)), dim3(256), 0, s,
-                       reinterpret_cast<unsigned long long *>(per_graph), (int64_t)G);
-    g->batch_voff = voff;
-    g->batch_ngraphs = G;
-    g->batch_per_graph = reinterpret_cast<unsigned long long *>(per_graph);
-    uint64_t glob = 0;
-    int st = validate_graph(*g, true, nullptr);
-    if (st == TC_OK) st = static_count(*g, per_vertex, &glob, sched, stats, false);
+    TC_TRY(batched_graph_create(G, voff, off, adj, nnz, nv_total, s, &g));
+    const int st = batched_graph_count(*g, per_graph, per_vertex, sched, stats);
     graph_destroy(g);
     return st;
 }

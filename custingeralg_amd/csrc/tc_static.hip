@@ -786,9 +786,11 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
     __shared__ uint32_t tcnt[NW][kWave];
     __shared__ unsigned long long red[NW];
     __shared__ int32_t wsum[NW];
-    // bitmap tiles (largest bin): words and their u16 prefixes share HB's space
-    constexpr bool kUseBitmap = LMAX >= 4096;
+    // bitmap tiles (largest bin only: its jobs are long enough to amortise
+    // building a bitmap over the tile's value range): words and their u16
+    // prefixes share HB's space
     constexpr bool kChunkedLong = LMAX < 4096;
+    constexpr bool kUseBitmap = LMAX >= 4096;
     constexpr int kBmWords = (16 * NBMAX) / 6;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;

@@ -217,6 +217,16 @@ int tc_batched_count(int32_t G, const int64_t *voff_dev, const int64_t *off_dev,
                      uint64_t *per_graph_dev, uint32_t *per_vertex_dev,
                      const tc_schedule *sched, tc_stats *stats, void *hip_stream);
 
+/* The same input as a read-only handle, validated once and counted any number
+ * of times (no counterpart in the reference: for callers that count the same
+ * batch of graphs repeatedly).  The device arrays must outlive the handle,
+ * which is released with tc_graph_destroy and cannot take batch updates. */
+int tc_batched_graph_create(int32_t G, const int64_t *voff_dev, const int64_t *off_dev,
+                            const int32_t *adj_dev, int64_t nnz, int32_t nv_total, void *hip_stream,
+                            tc_graph **out);
+int tc_batched_graph_count(tc_graph *g, uint64_t *per_graph_dev, uint32_t *per_vertex_dev,
+                           const tc_schedule *sched, tc_stats *stats);
+
 /* ----------------------------------------------------- load balancing -- */
 /* cusLoadBalance::LoadBalanceArray (include/load_balance.cuh:77-102):
  * work[i] = degree of verts[i] (device int32 ids, len entries); returns the

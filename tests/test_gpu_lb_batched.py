@@ -57,3 +57,23 @@ def test_batched_generation_and_counts():
     pg2, pv, _ = T.batched_count(voff_h, off_h, adj_h, per_vertex=True)
     assert np.array_equal(pg2, got)
     assert np.array_equal(pv, O.custinger_count(off_h, adj_h))
+
+
+def test_batched_handle_repeated_counts():
+    """tc_batched_graph_create / _count: one validation, several counts (also
+    with per-vertex output and another schedule), each equal to the oracle."""
+    n, ef, seeds = O.batched_spec(G=24, nmin=256, nmax=8192)
+    voff_h, off_h, adj_h = O.batched_graph(n, ef, seeds)
+    dvoff, doff, dadj = T.batched_graph_device(n, ef, seeds)
+    want_pv = O.custinger_count(off_h, adj_h)
+    want = np.array([int(want_pv[voff_h[i]:voff_h[i + 1]].astype(np.uint64).sum())
+                     for i in range(len(n))], np.uint64)
+    bg = T.BatchedGraph(dvoff, doff, dadj)
+    pg = T.DeviceArray(len(n), np.uint64)
+    pv = T.DeviceArray(int(voff_h[-1]), np.uint32)
+    for sched in (None, T.default_schedule(small_max=32, mid_max=256, job_items=1024),
+                  T.default_schedule(policy=T.POLICY_MERGE)):
+        bg.count(pg, pv, sched)
+        assert np.array_equal(pg.to_host(), want)
+        assert np.array_equal(pv.to_host(), want_pv)
+    bg.close()
