@@ -27,6 +27,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CUSTINGER_TC_LIB") or os.path.join(_HERE, "libcustinger_tc.so")
 
 TC_OK, TC_ERR_INVALID, TC_ERR_OOM, TC_ERR_HIP, TC_ERR_UNSUPPORTED, TC_ERR_CAPACITY = range(6)
+FMT_AUTO, FMT_DIMACS, FMT_SNAP, FMT_MTX = range(4)
+READ_AS_IS = 0x1
 TC_HOST, TC_DEVICE = 0, 1
 TC_GRAPH_ASSUME_SYMMETRIC, TC_GRAPH_NO_VALIDATE = 0x1, 0x2
 POLICY_FIXED, POLICY_LEARNED, POLICY_MERGE, POLICY_BSEARCH, POLICY_HASH = range(5)
@@ -46,7 +48,7 @@ EXPORTS = [
     "tc_batched_count", "tc_lb_partition", "tc_lb_degree_bins", "tc_gen_rmat_pairs",
     "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune", "tc_schedule_set_default",
-    "tc_batched_graph_create", "tc_batched_graph_count",
+    "tc_batched_graph_create", "tc_batched_graph_count", "tc_read_graph", "tc_host_free",
 ]
 
 
@@ -152,6 +154,9 @@ def lib():
         L.tc_schedule_set_default.argtypes = [C.POINTER(Schedule)]
         L.tc_batched_graph_create.argtypes = [i32, vp, vp, vp, i64, i32, vp, C.POINTER(vp)]
         L.tc_batched_graph_count.argtypes = [vp, vp, vp, C.POINTER(Schedule), C.POINTER(Stats)]
+        L.tc_read_graph.argtypes = [C.c_char_p, C.c_int, u32, C.POINTER(i32), C.POINTER(i64),
+                                    C.POINTER(C.POINTER(i64)), C.POINTER(C.POINTER(i32))]
+        L.tc_host_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -426,6 +431,24 @@ class BatchedGraph:
             self.close()
         except Exception:
             pass
+
+
+def read_graph(path: str, fmt: int = FMT_AUTO, as_is: bool = False):
+    """readGraphDIMACS / readGraphSNAP / readGraphMatrixMarket: a graph file as
+    a host CSR (off int64[nv+1], adj int32[nnz]); symmetrised, loop-free,
+    deduplicated and sorted unless as_is."""
+    nv, nnz = C.c_int32(0), C.c_int64(0)
+    po, pa = C.POINTER(C.c_int64)(), C.POINTER(C.c_int32)()
+    check(lib().tc_read_graph(os.fsencode(path), fmt, READ_AS_IS if as_is else 0, C.byref(nv),
+                              C.byref(nnz), C.byref(po), C.byref(pa)))
+    try:
+        off = np.ctypeslib.as_array(po, shape=(nv.value + 1,)).copy()
+        adj = (np.ctypeslib.as_array(pa, shape=(nnz.value,)).copy() if nnz.value
+               else np.zeros(0, np.int32))
+    finally:
+        lib().tc_host_free(C.cast(po, C.c_void_p))
+        lib().tc_host_free(C.cast(pa, C.c_void_p))
+    return off, adj
 
 
 # ------------------------------------------------------------ reference shape

@@ -238,6 +238,23 @@ int tc_lb_partition(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t 
  * counts of vertices with degree >= 64, in [16,64) and < 16 among verts. */
 int tc_lb_degree_bins(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t counts[3]);
 
+/* --------------------------------------------------------- graph files -- */
+/* readGraphDIMACS / readGraphSNAP / readGraphMatrixMarket, the external
+ * cuStinger utilities the reference drivers call (tests/cct.cpp:239-254,
+ * tests/cct_upd.cpp:210-219): DIMACS/METIS ".graph" adjacency lists (1-based),
+ * SNAP ".txt" edge lists (0-based, '#' comments) and MatrixMarket coordinate
+ * ".mtx" files (1-based; "symmetric" stores one triangle).  TC_FMT_AUTO picks
+ * the format from the name as the drivers do.  The result is a host CSR,
+ * symmetrised, without self-loops, deduplicated and row-sorted unless
+ * TC_READ_AS_IS (pairs as given, only sorted and deduplicated).  *off
+ * (int64[nv+1]) and *adj (int32[nnz]) are malloc'd by the library; release
+ * them with tc_host_free (or free). */
+enum { TC_FMT_AUTO = 0, TC_FMT_DIMACS = 1, TC_FMT_SNAP = 2, TC_FMT_MTX = 3 };
+enum { TC_READ_AS_IS = 0x1 };
+int tc_read_graph(const char *path, int format, uint32_t flags, int32_t *nv, int64_t *nnz,
+                  int64_t **off, int32_t **adj);
+int tc_host_free(void *p);
+
 /* ---------------------------------------------------------- generators -- */
 /* Counter-based R-MAT / uniform pair generators and CSR canonicaliser
  * (symmetrise, drop self-loops, dedup, sort) on the device.  The same

@@ -26,6 +26,8 @@
 #define CUSTINGER_TC_HPP
 
 #include <cstdint>
+#include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -326,6 +328,54 @@ inline void callDeviceAllTrianglesCSR(const vertexId_t nv, const length_t *d_off
                                  reinterpret_cast<uint32_t *>(outPutTriangles), nullptr, nullptr);
     tc_device_free(d64);
     tcCheck(st);
+}
+
+// readGraphDIMACS / readGraphSNAP / readGraphMatrixMarket with the drivers'
+// signatures (tests/cct.cpp:246-254): host CSR with int32 offsets, malloc'd
+// (the caller frees with free()).  The last flag of each is accepted and not
+// used: the result is always the symmetric, loop-free, deduplicated graph.
+inline void readGraphFile(const char *path, int format, length_t **off, vertexId_t **adj,
+                          length_t *nv, length_t *ne)
+{
+    int32_t n = 0;
+    int64_t m = 0, *o = nullptr;
+    int32_t *a = nullptr;
+    tcCheck(tc_read_graph(path, format, 0, &n, &m, &o, &a));
+    if (m > INT32_MAX) {
+        tc_host_free(o);
+        tc_host_free(a);
+        throw cuStingerError(TC_ERR_CAPACITY, "graph has more than 2^31-1 entries; use tc_read_graph");
+    }
+    length_t *o32 = static_cast<length_t *>(std::malloc(sizeof(length_t) * ((size_t)n + 1)));
+    if (!o32) {
+        tc_host_free(o);
+        tc_host_free(a);
+        throw cuStingerError(TC_ERR_OOM, "host allocation failed");
+    }
+    for (int64_t i = 0; i <= n; ++i) o32[i] = (length_t)o[i];
+    tc_host_free(o);
+    *off = o32;
+    *adj = a;
+    *nv = n;
+    *ne = (length_t)m;
+}
+inline void readGraphDIMACS(const char *path, length_t **off, vertexId_t **adj, length_t *nv,
+                            length_t *ne, bool isRmat)
+{
+    (void)isRmat;
+    readGraphFile(path, TC_FMT_DIMACS, off, adj, nv, ne);
+}
+inline void readGraphSNAP(const char *path, length_t **off, vertexId_t **adj, length_t *nv,
+                          length_t *ne, bool isRmat)
+{
+    (void)isRmat;
+    readGraphFile(path, TC_FMT_SNAP, off, adj, nv, ne);
+}
+inline void readGraphMatrixMarket(const char *path, length_t **off, vertexId_t **adj,
+                                  length_t *nv, length_t *ne, bool undirected)
+{
+    (void)undirected;
+    readGraphFile(path, TC_FMT_MTX, off, adj, nv, ne);
 }
 
 // ------------------------------------------------------ algorithm classes --

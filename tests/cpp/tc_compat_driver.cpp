@@ -256,11 +256,58 @@ static void insertion_test()
     custingTest.freecuStinger();
 }
 
+// readGraphDIMACS -> initializeCuStinger -> callDeviceAllTriangles, as the
+// drivers do with a file argument (tests/cct.cpp:246-277).
+static void file_test()
+{
+    const Csr c = rmat(9, 8, 5);
+    char path[] = "/tmp/tc_compat_XXXXXX.graph";
+    const int fd = mkstemps(path, 6);
+    CHECK(fd >= 0, "temporary file");
+    if (fd < 0) return;
+    FILE *f = fdopen(fd, "w");
+    std::fprintf(f, "%% generated\n%d %lld\n", c.nv, (long long)(c.adj.size() / 2));
+    for (int32_t v = 0; v < c.nv; ++v) {
+        for (int64_t k = c.off[(size_t)v]; k < c.off[(size_t)v + 1]; ++k)
+            std::fprintf(f, "%d ", c.adj[(size_t)k] + 1);
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+    length_t nv = 0, ne = 0, *off = nullptr;
+    vertexId_t *adj = nullptr;
+    readGraphDIMACS(path, &off, &adj, &nv, &ne, false);
+    std::remove(path);
+    CHECK(nv == c.nv && ne == (length_t)c.adj.size(), "file sizes");
+    cuStingerInitConfig cuInit;
+    cuInit.initState = eInitStateCSR;
+    cuInit.maxNV = nv + 1;
+    cuInit.csrNV = nv;
+    cuInit.csrNE = ne;
+    cuInit.csrOff = off;
+    cuInit.csrAdj = adj;
+    cuStinger custing(defaultInitAllocater, defaultUpdateAllocater);
+    custing.initializeCuStinger(cuInit);
+    triangle_t *d = (triangle_t *)allocDeviceArray(nv + 1, sizeof(triangle_t));
+    callDeviceAllTriangles(custing, d, 1, 128, 0, 16000, 128);
+    std::vector<triangle_t> h((size_t)nv);
+    copyArrayDeviceToHost(d, h.data(), nv, sizeof(triangle_t));
+    const std::vector<uint32_t> want = oracle_pv(c);
+    bool same = true;
+    for (int32_t i = 0; i < nv; ++i) same = same && (uint32_t)h[(size_t)i] == want[(size_t)i];
+    CHECK(same, "count of the graph read from a DIMACS file");
+    std::printf("file test: nv=%d ne=%d sum=%lld\n", nv, ne, (long long)tco_sum_triangle_array(h.data(), nv));
+    freeDeviceArray(d);
+    std::free(off);
+    std::free(adj);
+    custing.freecuStinger();
+}
+
 int main()
 {
     try {
         static_sweep();
         insertion_test();
+        file_test();
     } catch (const cuStingerError &e) {
         std::printf("FAIL exception: %s\n", e.what());
         return 2;
