@@ -106,6 +106,34 @@ struct BucketTable {
         }
         return w >= 0 ? s : -1;  // empty slots hold -1
     }
+    // Batched find: both buckets of every value are read before any is matched.
+    template <int U>
+    __device__ inline void find_n(const int32_t *w, int32_t *slot) const
+    {
+        int4 vx[U], vy[U];
+        uint32_t x[U], y[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x[u] = b1(w[u]);
+            y[u] = b2(w[u]);
+            vx[u] = HB[x[u]];
+            vy[u] = HB[y[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int32_t s = match(vx[u], x[u], w[u]);
+            if (s < 0) s = match(vy[u], y[u], w[u]);
+            if (s < 0 && w[u] >= 0 && vx[u].w >= 0 && vy[u].w >= 0) {
+                const uint32_t m = (1u << nbb) - 1;
+                for (uint32_t b = (x[u] + 1) & m;; b = (b + 1) & m) {
+                    const int4 v = HB[b];
+                    s = match(v, b, w[u]);
+                    if (s >= 0 || v.w < 0) break;
+                }
+            }
+            slot[u] = w[u] >= 0 ? s : -1;
+        }
+    }
     __device__ inline int32_t id_at(int32_t slot) const
     {
         return reinterpret_cast<const int32_t *>(HB)[slot];
@@ -113,23 +141,32 @@ struct BucketTable {
 };
 
 // Membership bitmap of a sorted owner tile over its value range [lo, lo+span):
-// bit (w - lo) is set for every tile entry and PF[i] counts the set bits of
-// words < i, so the position of w in the tile (its counter slot) is PF[word]
-// plus the set bits below w in its word.  One 4-byte LDS read per probe.
+// word i holds the bits of values lo+32i .. lo+32i+31 (x) and the number of
+// set bits in words < i (y), so the position of w in the tile (its counter
+// slot) is y plus the set bits below w in x.  One 8-byte LDS read per probe.
+// find_n probes a batch branch-free: all reads are issued before any is used
+// (out-of-range values read word 0 and are discarded).
 struct TileBitmap {
-    const uint32_t *BM;
-    const uint16_t *PF;
+    const uint2 *BM;
     int32_t lo;
     uint32_t span;
 
-    __device__ inline int32_t find(int32_t w) const
+    template <int U>
+    __device__ inline void find_n(const int32_t (&w)[U], int32_t (&slot)[U]) const
     {
-        const uint32_t off = (uint32_t)(w - lo);  // w < lo (and w = -1) wraps past span
-        if (off >= span) return -1;
-        const uint32_t x = BM[off >> 5];
-        const uint32_t bit = off & 31u;
-        if (!((x >> bit) & 1u)) return -1;
-        return (int32_t)PF[off >> 5] + __popc(x & ((1u << bit) - 1u));
+        uint2 x[U];
+        uint32_t off[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            off[u] = (uint32_t)(w[u] - lo);  // w < lo (and w = -1) wraps past span
+            x[u] = BM[off[u] < span ? (off[u] >> 5) : 0u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t b = off[u] & 31u;
+            const bool hit = off[u] < span && ((x[u].x >> b) & 1u);
+            slot[u] = hit ? (int32_t)(x[u].y + __popc(x[u].x & ((1u << b) - 1u))) : -1;
+        }
     }
 };
 
@@ -791,7 +828,7 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
     // prefixes share HB's space
     constexpr bool kChunkedLong = LMAX < 4096;
     constexpr bool kUseBitmap = LMAX >= 4096;
-    constexpr int kBmWords = (16 * NBMAX) / 6;
+    constexpr int kBmWords = (16 * NBMAX) / 8;
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     const int nthr = NW * kWave;
@@ -817,32 +854,31 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
         const int32_t nwords = (int32_t)((span + 31u) >> 5);
         const bool use_bm = kUseBitmap && nwords <= kBmWords;
-        uint32_t *BMw = reinterpret_cast<uint32_t *>(HB);
-        uint16_t *PF = reinterpret_cast<uint16_t *>(BMw + kBmWords);
-        const TileBitmap BT{BMw, PF, lo_v, span};
+        uint2 *BMw = reinterpret_cast<uint2 *>(HB);
+        const TileBitmap BT{BMw, lo_v, span};
         if (use_bm) {
-            for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = 0u;
+            for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
             if (ORIENTED)
                 for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < L; i += nthr) {
                 const uint32_t off = (uint32_t)(Sg[i] - lo_v);
-                atomicOr(&BMw[off >> 5], 1u << (off & 31u));
+                atomicOr(&BMw[off >> 5].x, 1u << (off & 31u));
             }
             __syncthreads();
-            // PF = exclusive prefix of the word popcounts (contiguous chunk per thread)
+            // .y = exclusive prefix of the word popcounts (contiguous chunk per thread)
             const int32_t per = (nwords + nthr - 1) / nthr;
             const int32_t k0 = threadIdx.x * per, k1 = min(nwords, k0 + per);
             int32_t mine = 0;
-            for (int32_t k = k0; k < k1; ++k) mine += __popc(BMw[k]);
+            for (int32_t k = k0; k < k1; ++k) mine += __popc(BMw[k].x);
             const int32_t incl = wave_incl_scan_i32(mine);
             if (lane == kWave - 1) wsum[wid] = incl;
             __syncthreads();
             int32_t run = incl - mine;
             for (int w = 0; w < wid; ++w) run += wsum[w];
             for (int32_t k = k0; k < k1; ++k) {
-                PF[k] = (uint16_t)run;
-                run += __popc(BMw[k]);
+                BMw[k].y = (uint32_t)run;
+                run += __popc(BMw[k].x);
             }
             __syncthreads();
         } else {
@@ -853,7 +889,26 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
             __syncthreads();
         }
-        auto lookup = [&](int32_t w) -> int32_t { return use_bm ? BT.find(w) : T.find(w); };
+        // batched probes: bitmap reads all issued before use; the hash table one by one
+        auto lookup_n = [&](auto &w, auto &slot) {
+            constexpr int n = (int)(sizeof(w) / sizeof(w[0]));
+            if (use_bm) {
+                BT.find_n(w, slot);
+            } else {
+                // lookups in flight: four in the small bins; two in the largest
+                // bin, where the table is the fallback of the bitmap and
+                // registers are scarce
+                constexpr int gmax = LMAX >= 4096 ? 2 : 4;
+                constexpr int g = n < gmax ? n : gmax;
+#pragma unroll
+                for (int u = 0; u < n; u += g) T.find_n<g>(&w[u], &slot[u]);
+            }
+        };
+        auto lookup = [&](int32_t w) -> int32_t {
+            int32_t ww1[1] = {w}, s1[1];
+            lookup_n(ww1, s1);
+            return s1[0];
+        };
         unsigned long long hits_job = 0;
 
         // tasks are dealt to the waves round-robin so short jobs still keep every wave busy
@@ -927,12 +982,13 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                             const int32_t k = k0 + u * kWave;
                             w[u] = (k < n) ? lst[k] : -1;
                         }
+                        int32_t slots[kBigUnroll];
+                        lookup_n(w, slots);
 #pragma unroll
                         for (int u = 0; u < kBigUnroll; ++u) {
-                            const int32_t slot = lookup(w[u]);
-                            if (slot >= 0) {
+                            if (slots[u] >= 0) {
                                 ++h;
-                                if (ORIENTED) atomicAdd(&C[slot], 1u);
+                                if (ORIENTED) atomicAdd(&C[slots[u]], 1u);
                             }
                         }
                     }
@@ -962,9 +1018,11 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                             jj[u] = j;
                         }
                     }
+                    int32_t slots[kBigUnroll];
+                    lookup_n(w, slots);
 #pragma unroll
                     for (int u = 0; u < kBigUnroll; ++u) {
-                        const int32_t slot = lookup(w[u]);
+                        const int32_t slot = slots[u];
                         if (ORIENTED && slot >= 0) atomicAdd(&C[slot], 1u);
                         const uint32_t hc = (uint32_t)__popcll(__ballot(slot >= 0));
                         if (jj[u] != cur_j) {
@@ -1021,9 +1079,11 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                         }
                     }
                     if (phase == 0) {
+                        int32_t slots[kUnroll];
+                        lookup_n(ww, slots);
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
-                            const int32_t slot = lookup(ww[u]);
+                            const int32_t slot = slots[u];
                             if (!ok[u]) continue;
                             if (jj[u] != myj) {
                                 if (myhits) atomicAdd(&tcnt[wid][myj], myhits);
