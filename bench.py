@@ -305,7 +305,10 @@ def bench_batched(T, args) -> dict:
             "config": {"workload": "batched static TC, 4096 R-MAT graphs, n log-uniform "
                                    "[1k,64k], ef U{4..32}, seeds 1000+i",
                        "nnz": nnz, "sum_per_graph": int(per_graph.sum()),
-                       "bins_ms": st["ms_bin"], "ms_prepare": st["ms_prepare"]},
+                       "bins_ms": st["ms_bin"], "ms_prepare": st["ms_prepare"],
+                       "bins": {"tasks": st["tasks_bin"], "jobs": st["jobs_bin"],
+                                "items": st["items_bin"], "stream_bytes": st["stream_bytes_bin"],
+                                "strategy_tasks": st["strategy_tasks"]}},
             "roofline": None, "cpu_baseline": None}
 
 

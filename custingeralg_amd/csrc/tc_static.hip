@@ -84,10 +84,12 @@ struct BucketTable {
         for (uint32_t b = (x + 1) & m;; b = (b + 1) & m)
             if (claim(b, w)) return;
     }
+    // Slot of w in bucket b or -1; arithmetic only (ids in a bucket are distinct).
     __device__ inline int32_t match(const int4 v, uint32_t b, int32_t w) const
     {
-        const int32_t base = (int32_t)(b << 2);
-        return v.x == w ? base : v.y == w ? base + 1 : v.z == w ? base + 2 : v.w == w ? base + 3 : -1;
+        const int32_t m0 = v.x == w, m1 = v.y == w, m2 = v.z == w, m3 = v.w == w;
+        const int32_t idx = m1 + 2 * m2 + 3 * m3;
+        return (m0 | m1 | m2 | m3) ? (int32_t)(b << 2) + idx : -1;
     }
     // Slot of w, or -1 (w < 0 is never present).
     __device__ inline int32_t find(int32_t w) const
@@ -163,9 +165,11 @@ struct TileBitmap {
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+            // bitwise tests, no short-circuit: keeps the batch free of branches
             const uint32_t b = off[u] & 31u;
-            const bool hit = off[u] < span && ((x[u].x >> b) & 1u);
-            slot[u] = hit ? (int32_t)(x[u].y + __popc(x[u].x & ((1u << b) - 1u))) : -1;
+            const uint32_t hit = (uint32_t)(off[u] < span) & ((x[u].x >> b) & 1u);
+            const int32_t pos = (int32_t)(x[u].y + __popc(x[u].x & ((1u << b) - 1u)));
+            slot[u] = hit ? pos : -1;
         }
     }
 };
@@ -979,6 +983,8 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                         int32_t w[kBigUnroll];
 #pragma unroll
                         for (int u = 0; u < kBigUnroll; ++u) {
+                            // masked loads: a slot past the end of the task is
+                            // skipped by the whole wave
                             const int32_t k = k0 + u * kWave;
                             w[u] = (k < n) ? lst[k] : -1;
                         }
