@@ -758,6 +758,22 @@ __global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restr
     }
 }
 
+// gid[r] = graph holding vertex order[r] (or r): last g with voff[g] <= vertex.
+__global__ void k_graph_of(const int64_t *__restrict__ voff, int32_t G,
+                           const int32_t *__restrict__ order, int32_t nv, int32_t *__restrict__ gid)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = order ? order[r] : r;
+        int32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (voff[mid] <= v) lo = mid; else hi = mid - 1;
+        }
+        gid[r] = lo;
+    }
+}
+
 // ---------------------------------------------------------- count kernel --
 
 struct CountArgs {
@@ -776,8 +792,7 @@ struct CountArgs {
     int32_t skip_lower;  // CSR convention: entries with dest < src are skipped
     // batched graphs: graph g owns vertices [gvoff[g], gvoff[g+1]); a job's
     // hits are added to per_graph[g] of its owner (triangles never span graphs)
-    const int64_t *gvoff;
-    int32_t ngraphs;
+    const int32_t *gid;  // graph of each owner (rank or vertex id, as jobs name them)
     unsigned long long *per_graph;
 };
 
@@ -1198,14 +1213,8 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         if (threadIdx.x == 0 && hjob)
             atomicAdd(&A.out[owner_v], ORIENTED ? (uint32_t)(2 * hjob) : (uint32_t)hjob);
         hits_all += (threadIdx.x == 0) ? hjob : 0ull;
-        if (A.per_graph && threadIdx.x == 0 && hjob) {
-            int32_t lo = 0, hi = A.ngraphs - 1;  // last graph whose first vertex <= owner
-            while (lo < hi) {
-                const int32_t mid = (lo + hi + 1) >> 1;
-                if (A.gvoff[mid] <= owner_v) lo = mid; else hi = mid - 1;
-            }
-            atomicAdd(&A.per_graph[lo], (ORIENTED ? 6ull : 1ull) * hjob);
-        }
+        if (A.per_graph && threadIdx.x == 0 && hjob)
+            atomicAdd(&A.per_graph[A.gid[J.owner]], (ORIENTED ? 6ull : 1ull) * hjob);
         __syncthreads();
     }
     if (threadIdx.x == 0 && hits_all) atomicAdd(A.global, (ORIENTED ? 6ull : 1ull) * hits_all);
@@ -1594,9 +1603,17 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.policy = sched.policy;
     A.ratio_x16 = sched.ratio_x16 > 0 ? sched.ratio_x16 : 64;
     A.skip_lower = skip_lower ? 1 : 0;
-    A.gvoff = g.batch_voff;
-    A.ngraphs = g.batch_ngraphs;
     A.per_graph = g.batch_per_graph;
+    A.gid = nullptr;
+    if (g.batch_per_graph) {
+        // graph of every owner, found once here instead of once per job
+        TC_TRY(g.ov[11].ensure((size_t)std::max(nv, 1) * sizeof(int32_t)));
+        hipLaunchKernelGGL(k_graph_of, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, g.batch_voff,
+                           g.batch_ngraphs, oriented ? g.order.as<int32_t>() : nullptr, nv,
+                           g.ov[11].as<int32_t>());
+        TC_HIP(hipGetLastError());
+        A.gid = g.ov[11].as<int32_t>();
+    }
     A.table = nullptr;
     A.table_dim = 0;
     if (sched.policy == TC_POLICY_LEARNED) {
