@@ -129,6 +129,8 @@ struct Graph {
     DevBuf order, rnk, tasks;
     DevBuf ov[12];
     int64_t ntasks = 0;  // entries of the task array (oriented layout)
+    int64_t nplus = 0;   // entries of the plus array (oriented layout)
+    DevBuf pcnt;         // per N+ entry: hits of the task it became (summed per vertex)
     bool oriented_views = false;
     DevBuf bsrc, bdst, bflag, bkeys, bkeys2, bvals2, boffb, bdel;
     // streaming: per-entry "in the current batch" flags parallel to the pool,

@@ -758,6 +758,32 @@ __global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restr
     }
 }
 
+// out[order[r]] += 2 * (sum of pcnt over N+(r)): the task-vertex share of the
+// oriented count, one segmented wave sum per 64 entries of the plus array.
+__global__ void k_plus_row_sum(const int64_t *__restrict__ pcoff, const int32_t *__restrict__ pcrow,
+                               const uint32_t *__restrict__ pcnt, const int32_t *__restrict__ order,
+                               int32_t nv, int64_t nplus, uint32_t *__restrict__ out)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nplus, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        const int64_t e = c * kWave + lane;
+        const int32_t r = wave_row_of(pcoff, pcrow, nv, c, nplus);
+        uint32_t x = e < nplus ? pcnt[e] : 0u;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, kWave);
+            const int32_t ry = __shfl_up(r, d, kWave);
+            if (lane >= d && ry == r) x += y;
+        }
+        const int32_t rn = __shfl_down(r, 1, kWave);
+        const bool tail = lane == kWave - 1 || rn != r || e + 1 >= nplus;
+        if (e < nplus && tail && x) atomicAdd(&out[order[r]], 2u * x);
+    }
+}
+
 // gid[r] = graph holding vertex order[r] (or r): last g with voff[g] <= vertex.
 __global__ void k_graph_of(const int64_t *__restrict__ voff, int32_t G,
                            const int32_t *__restrict__ order, int32_t nv, int32_t *__restrict__ gid)
@@ -782,6 +808,7 @@ struct CountArgs {
     const TaskRec *tasks;  // oriented: task vertex and suffix of each task
     const int64_t *toff;   // oriented: first task of each owner
     const int32_t *order;  // oriented: vertex of each rank (outputs are by vertex)
+    uint32_t *pcnt;        // oriented: per N+ entry, the hits of its task
     uint32_t *out;
     unsigned long long *global;
     unsigned long long *strat;  // [3] tasks per strategy (may be null)
@@ -1186,7 +1213,15 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
             const uint32_t c = tcnt[wid][lane] + mcount;
             if (valid && c) {
                 hits_job += c;
-                if (ORIENTED) atomicAdd(&A.out[A.order[tv]], 2u * c);
+                // the task's hits go to its N+ entry (the one before its suffix),
+                // summed per vertex after the count: a store, not a global atomic
+                // (a tiled owner meets the task once per tile)
+                if (ORIENTED) {
+                    if (tiled)
+                        atomicAdd(&A.pcnt[sb - 1], c);
+                    else
+                        A.pcnt[sb - 1] = c;
+                }
             }
             wave_sync();
         }
@@ -1461,6 +1496,7 @@ int prepare_views(Graph &g, bool oriented)
         return TC_ERR_INVALID;
     }
     g.ntasks = ntasks;
+    g.nplus = nplus;
     g.oriented_views = true;
     return TC_OK;
 }
@@ -1595,6 +1631,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.adj = adj;
     A.vi = vi;
     A.tasks = tasks;
+    A.pcnt = nullptr;
     A.toff = oriented ? coff : nullptr;
     A.order = oriented ? g.order.as<int32_t>() : nullptr;
     A.out = outp;
@@ -1621,24 +1658,38 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         A.table = learned_table_device(&dim);
         A.table_dim = dim;
     }
+    const int64_t nplus = oriented ? g.nplus : 0;
+    if (oriented) {
+        TC_TRY(g.pcnt.ensure((size_t)std::max<int64_t>(nplus, 1) * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nplus, 256, 65536)), dim3(256), 0, s,
+                           g.pcnt.as<uint32_t>(), nplus, 0u);
+        A.pcnt = g.pcnt.as<uint32_t>();
+    }
     const Job *jl[3] = {j0, j1, j2};
     for (int b = 0; b < 3; ++b) {
         TC_HIP(hipEventRecord(g.ev[3 + b], s));
         TC_TRY(launch_bin(b, oriented, jl[b], jt[b], A, s));
     }
-    TC_HIP(hipEventRecord(g.ev[6], s));
-    TC_HIP(hipEventSynchronize(g.ev[6]));
+    TC_HIP(hipEventRecord(g.ev[6], s));  // end of the bin kernels
+    if (oriented && nplus > 0) {
+        hipLaunchKernelGGL(k_plus_row_sum, dim3(grid_for(ceil_div(nplus, kWave), 4, 1 << 16)),
+                           dim3(256), 0, s, g.ov[6].as<int64_t>(), g.ov[10].as<int32_t>(),
+                           g.pcnt.as<uint32_t>(), g.order.as<int32_t>(), nv, nplus, outp);
+        TC_HIP(hipGetLastError());
+    }
+    TC_HIP(hipEventRecord(g.ev[7], s));
+    TC_HIP(hipEventSynchronize(g.ev[7]));
 
     unsigned long long hacc[16];
     TC_HIP(hipMemcpy(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost));
     if (global) *global = hacc[0];
     if (stats) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[6]);
+        (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[7]);
         st.ms_total = ms;
         (void)hipEventElapsedTime(&ms, g.ev[0], g.ev[2]);
         st.ms_prepare = ms;
-        (void)hipEventElapsedTime(&ms, g.ev[3], g.ev[6]);
+        (void)hipEventElapsedTime(&ms, g.ev[3], g.ev[7]);
         st.ms_count = ms;
         for (int b = 0; b < 3; ++b) {
             (void)hipEventElapsedTime(&ms, g.ev[3 + b], g.ev[4 + b]);
