@@ -25,6 +25,8 @@ namespace tc {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kBigRow = 4096;   // rows this long are compacted / merged by a whole block
+constexpr int kRowThreads = 1024;
 
 __device__ inline int64_t find_pos(const int32_t *__restrict__ pool, int64_t base, int32_t n,
                                    int32_t key)
@@ -176,6 +178,7 @@ __global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n,
         const int32_t v = queue[q];
         const int64_t base = boff[v];
         const int32_t l = len[v];
+        if (l >= kBigRow) continue;  // k_compact_rows_block
         int32_t w = 0;
         for (int32_t r0 = 0; r0 < l; r0 += kWave) {
             const int32_t r = r0 + lane;
@@ -196,6 +199,53 @@ __global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n,
         }
         for (int32_t r = w + lane; r < l; r += kWave) flag[base + r] = 0;
         if (lane == 0) len[v] = w;
+    }
+}
+
+// k_compact_rows for rows of at least kBigRow entries: one block per row,
+// chunks of blockDim entries, block-wide prefix of the kept entries.
+__global__ __launch_bounds__(kRowThreads) void k_compact_rows_block(
+    const int32_t *__restrict__ queue, int64_t n, const int64_t *__restrict__ boff,
+    int32_t *__restrict__ len, int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
+{
+    __shared__ int32_t wsum[kRowThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const int32_t v = queue[q];
+        const int32_t l = len[v];
+        if (l < kBigRow) continue;
+        const int64_t base = boff[v];
+        int32_t w = 0;
+        for (int32_t r0 = 0; r0 < l; r0 += blockDim.x) {
+            const int32_t r = r0 + threadIdx.x;
+            int32_t x = 0;
+            bool keep = false;
+            if (r < l) {
+                x = pool[base + r];
+                keep = flag[base + r] == 0;
+            }
+            const uint64_t m = __ballot(keep);
+            if (lane == 0) wsum[wid] = __popcll(m);
+            __syncthreads();  // every read of the chunk is done before any write
+            int32_t before = 0, total = 0;
+            for (int k = 0; k < nw; ++k) {
+                const int32_t c = wsum[k];
+                before += k < wid ? c : 0;
+                total += c;
+            }
+            if (keep) {
+                const int32_t dst = w + before + __popcll(m & lt);
+                pool[base + dst] = x;
+                flag[base + dst] = 0;
+            }
+            w += total;
+            __syncthreads();
+        }
+        for (int32_t r = w + threadIdx.x; r < l; r += blockDim.x) flag[base + r] = 0;
+        if (threadIdx.x == 0) len[v] = w;
+        __syncthreads();
     }
 }
 
@@ -279,9 +329,10 @@ __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__re
     for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; s < nseg;
          s += wstride) {
         const int32_t v = sv[s];
+        const int32_t lo = len[v];
+        if (lo >= kBigRow) continue;  // k_merge_rows_block
         const int64_t b0 = sbeg[s];
         const int32_t cn = (int32_t)(((s + 1 < nseg) ? sbeg[s + 1] : n) - b0);
-        const int32_t lo = len[v];
         const int64_t ob = boff[v];
         const bool move = need[s] > 0;
         const int64_t db = move ? pool_used + nscan[s] : ob;
@@ -325,6 +376,8 @@ __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__re
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                // shifts never decrease with j: below this chunk nothing moves
+                if (!__ballot(j < lo && a > 0)) break;
                 if (j < lo && a > 0) {
                     pool[ob + j + a] = y;
                     flag[ob + j + a] = 0;
@@ -348,6 +401,88 @@ __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__re
                 cap[v] = need[s];
             }
         }
+    }
+}
+
+// k_merge_rows for rows of at least kBigRow entries: the same three phases
+// with a whole block per row (block barriers between reads and writes).
+__global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
+    const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg, int64_t n,
+    const uint64_t *__restrict__ keys, int bv, const int32_t *__restrict__ need,
+    const int64_t *__restrict__ nscan, int64_t pool_used, int64_t *__restrict__ boff,
+    int32_t *__restrict__ len, int32_t *__restrict__ cap, int32_t *__restrict__ pool,
+    uint8_t *__restrict__ flag, int64_t *__restrict__ newpos)
+{
+    const uint64_t m = (1ull << bv) - 1;
+    const int32_t B = blockDim.x;
+    for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const int32_t v = sv[s];
+        const int32_t lo = len[v];
+        if (lo < kBigRow) continue;
+        const int64_t b0 = sbeg[s];
+        const int32_t cn = (int32_t)(((s + 1 < nseg) ? sbeg[s + 1] : n) - b0);
+        const int64_t ob = boff[v];
+        const bool move = need[s] > 0;
+        const int64_t db = move ? pool_used + nscan[s] : ob;
+        for (int32_t i = threadIdx.x; i < cn; i += B) {
+            const int32_t x = (int32_t)(keys[b0 + i] & m);
+            int32_t a = 0, b = lo;
+            while (a < b) {
+                const int32_t mid = (a + b) >> 1;
+                if (pool[ob + mid] < x) a = mid + 1; else b = mid;
+            }
+            newpos[b0 + i] = db + i + a;
+        }
+        __syncthreads();
+        if (move) {
+            for (int32_t j = threadIdx.x; j < lo; j += B) {
+                const int32_t y = pool[ob + j];
+                int32_t a = 0, b = cn;
+                while (a < b) {
+                    const int32_t mid = (a + b) >> 1;
+                    if ((int32_t)(keys[b0 + mid] & m) < y) a = mid + 1; else b = mid;
+                }
+                pool[db + j + a] = y;
+                flag[db + j + a] = 0;
+            }
+        } else {
+            const int32_t top = ((lo + B - 1) / B) * B;
+            for (int32_t j0 = top - B; j0 >= 0; j0 -= B) {
+                const int32_t j = j0 + threadIdx.x;
+                int32_t y = 0, a = 0;
+                if (j < lo) {
+                    y = pool[ob + j];
+                    int32_t lo2 = 0, hi2 = cn;
+                    while (lo2 < hi2) {
+                        const int32_t mid = (lo2 + hi2) >> 1;
+                        if ((int32_t)(keys[b0 + mid] & m) < y) lo2 = mid + 1; else hi2 = mid;
+                    }
+                    a = lo2;
+                }
+                // every read of the chunk is done before any write; below a
+                // chunk without shifts nothing moves
+                if (!__syncthreads_or(j < lo && a > 0)) break;
+                if (j < lo && a > 0) {
+                    pool[ob + j + a] = y;
+                    flag[ob + j + a] = 0;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        for (int32_t i = threadIdx.x; i < cn; i += B) {
+            const int64_t d = newpos[b0 + i];
+            pool[d] = (int32_t)(keys[b0 + i] & m);
+            flag[d] = 1;
+        }
+        if (threadIdx.x == 0) {
+            len[v] = lo + cn;
+            if (move) {
+                boff[v] = db;
+                cap[v] = need[s];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -767,6 +902,10 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
             hipLaunchKernelGGL(k_compact_rows, dim3(grid_for((int64_t)hq, 4, 65536)), dim3(256), 0, s,
                                queue, (int64_t)hq, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                                g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
+            hipLaunchKernelGGL(k_compact_rows_block, dim3(grid_for((int64_t)hq, 1, 1024)),
+                               dim3(kRowThreads), 0, s, queue, (int64_t)hq, g.st.boff.as<int64_t>(),
+                               g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
+                               g.flagpool.as<uint8_t>());
             hipLaunchKernelGGL(k_unmark, dim3(grid_for((int64_t)hq, 256, 65536)), dim3(256), 0, s,
                                queue, (int64_t)hq, g.vmark.as<int32_t>());
             TC_HIP(hipGetLastError());
@@ -840,6 +979,11 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
             hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(nseg, 4, 65536)), dim3(256), 0, s, sv, sbeg,
                                nseg, nd, dk, bv, need, nscan, g.st.pool_used, g.st.boff.as<int64_t>(),
                                g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
+                               g.flagpool.as<uint8_t>(), newpos);
+            hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(nseg, 1, 1024)), dim3(kRowThreads), 0,
+                               s, sv, sbeg, nseg, nd, dk, bv, need, nscan, g.st.pool_used,
+                               g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                               g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
                                g.flagpool.as<uint8_t>(), newpos);
             TC_HIP(hipGetLastError());
             g.st.pool_used += grow;

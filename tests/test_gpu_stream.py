@@ -220,3 +220,75 @@ def test_batch_prepare_matches_oracle(seed):
     e = T.batch_prepare(g, np.zeros(0, np.int32), np.zeros(0, np.int32))
     assert len(e[0]) == 0 and not e[3].any()
     g.close()
+
+
+def _hub_graph(n, hubs, hub_deg, seed):
+    """Background R-MAT pairs plus a few hub rows of hub_deg neighbours each."""
+    rng = np.random.default_rng(seed)
+    u, v = O.rmat_pairs(13, 20000, seed)
+    hu = np.repeat(np.arange(hubs, dtype=np.int32), hub_deg)
+    hv = np.concatenate([rng.choice(np.arange(hubs, n, dtype=np.int32), hub_deg, replace=False)
+                         for _ in range(hubs)])
+    a = np.concatenate([u % n, hu]).astype(np.int32)
+    b = np.concatenate([v % n, hv]).astype(np.int32)
+    return O.apply_batch_csr(n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32),
+                             ins_u=a, ins_v=b, del_u=np.zeros(0, np.int32),
+                             del_v=np.zeros(0, np.int32))
+
+
+@pytest.mark.parametrize("slack", [0, 100])
+def test_hub_rows_merge_and_compact(slack):
+    """Rows of several thousand entries (the block-per-row merge and compaction):
+    inserts spread over a hub row, inserts that all sort after its entries, inserts
+    that all sort before them, and deletions from it; store and counts against the
+    oracle after every batch."""
+    n = 16384
+    off, adj = _hub_graph(n, 3, 6000, 5)
+    assert np.diff(off).max() >= 6000
+    rng = np.random.default_rng(9)
+    g = T.cuStinger(off, adj, slack_pct=slack)
+    delta = T.DeviceArray(n, np.uint32).zero()
+    base = O.custinger_count(off, adj)
+    total = int(base.astype(np.uint64).sum())
+    cur_off, cur_adj = off, adj
+    plans = ["spread", "after", "before", "delete", "mixed"]
+    for bi, plan in enumerate(plans):
+        hub = bi % 3
+        row = cur_adj[cur_off[hub]:cur_off[hub + 1]]
+        if plan == "spread":
+            v = rng.integers(3, n, 3000).astype(np.int32)
+            u = np.full(len(v), hub, np.int32)
+            dl = np.zeros(len(v), np.uint8)
+        elif plan == "after":
+            hi = int(row.max()) + 1
+            v = np.arange(hi, min(n, hi + 500), dtype=np.int32)
+            u = np.full(len(v), hub, np.int32)
+            dl = np.zeros(len(v), np.uint8)
+        elif plan == "before":
+            lo = int(row.min())
+            v = np.arange(3, lo, dtype=np.int32)[:400]
+            u = np.full(len(v), hub, np.int32)
+            dl = np.zeros(len(v), np.uint8)
+        elif plan == "delete":
+            v = rng.choice(row, min(len(row), 2500), replace=False).astype(np.int32)
+            u = np.full(len(v), hub, np.int32)
+            dl = np.ones(len(v), np.uint8)
+        else:
+            vi = rng.integers(3, n, 1500).astype(np.int32)
+            vd = rng.choice(row, 1500, replace=False).astype(np.int32)
+            v = np.concatenate([vi, vd])
+            u = np.full(len(v), hub, np.int32)
+            dl = np.concatenate([np.zeros(1500, np.uint8), np.ones(1500, np.uint8)])
+        keep = u != v
+        u, v, dl = u[keep], v[keep], dl[keep]
+        dg, _ = T.batch_apply(g, u, v, dl, delta)
+        total += dg
+        want_off, want_adj = _host_apply(n, cur_off, cur_adj, u, v, dl)
+        got_off, got_adj = g.to_csr()
+        assert np.array_equal(got_off, want_off), f"batch {bi} ({plan}): offsets"
+        assert np.array_equal(got_adj, want_adj), f"batch {bi} ({plan}): adjacency"
+        cur_off, cur_adj = want_off, want_adj
+        now = O.custinger_count(cur_off, cur_adj)
+        assert np.array_equal(base + delta.to_host(), now), f"batch {bi} ({plan}): per-vertex"
+        assert total == int(now.astype(np.uint64).sum()), f"batch {bi} ({plan}): global"
+    g.close()
