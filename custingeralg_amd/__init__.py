@@ -62,7 +62,7 @@ class Schedule(C.Structure):
     _fields_ = [("method", C.c_int32), ("policy", C.c_int32), ("small_max", C.c_int32),
                 ("mid_max", C.c_int32), ("job_items", C.c_int32), ("ratio_x16", C.c_int32),
                 ("persistent", C.c_int32), ("flush_items", C.c_int32),
-                ("reserved", C.c_int32 * 8)]
+                ("hub_degree", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class Stats(C.Structure):

@@ -127,7 +127,7 @@ struct Graph {
     // pairs (a, position of b in the rank-sorted N+(a)) for a in N-(b);
     // ov = scratch of the orientation
     DevBuf order, rnk, tasks;
-    DevBuf ov[12];
+    DevBuf ov[13];
     int64_t ntasks = 0;  // entries of the task array (oriented layout)
     int64_t nplus = 0;   // entries of the plus array (oriented layout)
     DevBuf pcnt;         // per N+ entry: hits of the task it became (summed per vertex)
@@ -150,8 +150,10 @@ struct Graph {
     int ensure_events();
 };
 
-// Builds coff, crow and vinfo (and padj for the oriented layout) for the store.
-int prepare_views(Graph &g, bool oriented);
+// Builds coff, crow and vinfo (and padj for the oriented layout) for the store;
+// hub_degree > 0 overrides the degree from which owners build their task
+// lists from their own rows.
+int prepare_views(Graph &g, bool oriented, int32_t hub_degree = 0);
 // Fills vinfo with the store's rows as given (direct layout, dm=0).
 int build_direct_vinfo(Graph &g);
 // Validation of sortedness/range, self-loop count, optional symmetry check.
@@ -166,6 +168,36 @@ int graph_borrow_csr(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_
                      hipStream_t s, Graph **out);
 void graph_destroy(Graph *g);
 tc_schedule default_schedule();
+
+// ------------------------------------------------------- reductions --
+template <class T> __device__ inline T wave_sum(T x)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+// Adds the block's sums of x[0..N) to dst[0..N) with one atomic per nonzero
+// sum per block (device-scope atomics on one address serialise, so a grid of
+// many waves must not issue one each).  Every thread of the block calls it;
+// blockDim.x is a multiple of the wave size, at most 1024.
+template <int N>
+__device__ inline void block_add_u64(const unsigned long long (&x)[N], unsigned long long *dst)
+{
+    __shared__ unsigned long long part[16][N];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const unsigned long long t = wave_sum<unsigned long long>(x[q]);
+        if (lane == 0) part[wid][q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += part[w][threadIdx.x];
+        if (t) atomicAdd(&dst[threadIdx.x], t);
+    }
+}
 
 // Hash used by the LDS tables (multiplicative, table size 2^bits).
 __host__ __device__ inline uint32_t lds_hash(int32_t w, int bits)

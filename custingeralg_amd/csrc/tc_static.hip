@@ -194,13 +194,6 @@ __device__ inline int32_t wave_incl_scan_i32(int32_t x)
     return x;
 }
 
-template <class T> __device__ inline T wave_sum(T x)
-{
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
-    return x;
-}
-
 __host__ __device__ inline int ceil_log2_i(int64_t x)
 {
     int r = 0;
@@ -322,13 +315,26 @@ __global__ void k_orient_dm(const int64_t *__restrict__ coff, const int32_t *__r
 
 // Sort keys of the rank order: the degree, with the vertex id as payload (the
 // LSD radix sort is stable, so ties stay in id order: rank = (degree, id)).
+// sc[0] (zero on entry) receives the maximum degree, which bounds the key bits.
 __global__ void k_rank_keys(const int32_t *__restrict__ len, int32_t nv, uint64_t *__restrict__ keys,
-                            int32_t *__restrict__ ids)
+                            int32_t *__restrict__ ids, unsigned long long *__restrict__ sc)
 {
+    __shared__ unsigned long long part[16];
+    unsigned long long m = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
          v += (int64_t)gridDim.x * blockDim.x) {
-        keys[v] = (uint64_t)(uint32_t)len[v];
+        const uint32_t d = (uint32_t)len[v];
+        keys[v] = (uint64_t)d;
         ids[v] = (int32_t)v;
+        m = max(m, (unsigned long long)d);
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, d, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w) m = max(m, part[w]);
+        if (m) atomicMax(&sc[0], m);
     }
 }
 
@@ -534,8 +540,11 @@ __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int3
                                    const int32_t *__restrict__ padj, const VInfo *__restrict__ vi,
                                    const int64_t *__restrict__ toff, int32_t nv, int64_t nplus,
                                    TaskRec *__restrict__ tasks, int32_t *__restrict__ cursor,
-                                   int32_t *__restrict__ bad)
+                                   unsigned long long *__restrict__ sc, int32_t *__restrict__ bad)
 {
+    // owners from rank sc[1] up are the hubs, handled by k_orient_transpose_hubs
+    const int32_t hub_lo = (int32_t)sc[1];
+    unsigned long long nhub = 0;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nchunks = ceil_div(nplus, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
@@ -545,21 +554,114 @@ __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int3
         const int32_t r = wave_row_of(pcoff, pcrow, nv, c, nplus);
         if (e >= nplus) continue;
         const int32_t w = padj[e];
+        if (w >= hub_lo) {
+            ++nhub;  // filled by k_orient_transpose_hubs
+            continue;
+        }
         const int32_t slot = atomicAdd(&cursor[w], 1);
         if (slot < vi[w].dm)
             tasks[toff[w] + slot] = TaskRec{r, (int32_t)(pcoff[r + 1] - e - 1), e + 1};
         else
             atomicOr(bad, 1);  // rows are not symmetric
     }
+    const unsigned long long part[1] = {nhub};
+    block_add_u64<1>(part, sc + 2);
 }
 
-// Every N- part must have been filled exactly (holds for symmetric rows).
-__global__ void k_check_cursor(const int32_t *__restrict__ cursor, const VInfo *__restrict__ vi,
-                               int32_t nv, int32_t *__restrict__ bad)
+// Last rank r in [lo, hi] whose rows start at or before x in the degree
+// prefix pcoff[r] + toff[r] (= sum of the degrees of the ranks below r).
+__device__ inline int32_t hub_of(const int64_t *__restrict__ pcoff, const int64_t *__restrict__ toff,
+                                 int64_t x, int32_t lo, int32_t hi)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+    while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo + 1) >> 1);
+        if (pcoff[mid] + toff[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Task lists of the hubs (ranks from sc[1] up), built from the hubs' own store
+// rows instead of by appends: the k-th entry of the row of hub w that is
+// ranked below w is its task number k - (plus entries before it), and the
+// task's suffix starts after w's position in N+(a), found by binary search.
+// A hub gets an append from every part of the plus array, so its cursor would
+// serialise up to deg(w) atomics on one address.
+__global__ void k_orient_transpose_hubs(
+    const int64_t *__restrict__ pcoff, const int64_t *__restrict__ toff,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ rnk,
+    const int64_t *__restrict__ ocoff, const int64_t *__restrict__ boff,
+    const int32_t *__restrict__ pool, const uint64_t *__restrict__ mask,
+    const int64_t *__restrict__ cscan, const int32_t *__restrict__ padj, int32_t nv,
+    const unsigned long long *__restrict__ sc, TaskRec *__restrict__ tasks,
+    int32_t *__restrict__ bad)
+{
+    const int32_t hub_lo = (int32_t)sc[1];
+    if (hub_lo >= nv) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t e_lo = pcoff[hub_lo] + toff[hub_lo];
+    const int64_t n = pcoff[nv] + toff[nv] - e_lo;  // store entries of the hub rows
+    const int64_t nchunks = ceil_div(n, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        const int64_t i0 = c * kWave, ilast = min(i0 + kWave, n) - 1;
+        int32_t wf = 0, wl = 0;
+        if (lane == 0) wf = hub_of(pcoff, toff, e_lo + i0, hub_lo, nv - 1);
+        if (lane == 1) wl = hub_of(pcoff, toff, e_lo + ilast, hub_lo, nv - 1);
+        wf = __shfl(wf, 0, kWave);
+        wl = __shfl(wl, 1, kWave);
+        const int64_t i = i0 + lane;
+        if (i > ilast) continue;
+        const int32_t w = wf == wl ? wf : hub_of(pcoff, toff, e_lo + i, wf, wl);
+        const int64_t k = e_lo + i - (pcoff[w] + toff[w]);
+        const int32_t v = order[w];
+        const int64_t rs = ocoff[v];
+        const int64_t e = rs + k;  // compact store-order index of the entry
+        if ((mask[e >> 6] >> (e & 63)) & 1ull) continue;  // ranked above w: not a task
+        const int64_t slot = k - (plus_prefix(cscan, mask, e) - plus_prefix(cscan, mask, rs));
+        const int32_t a = rnk[pool[boff[v] + k]];
+        const int64_t pa = pcoff[a], pn = pcoff[a + 1] - pa;
+        int64_t lo = 0, hi = pn;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (padj[pa + mid] < w) lo = mid + 1; else hi = mid;
+        }
+        if (lo < pn && padj[pa + lo] == w)
+            tasks[toff[w] + slot] = TaskRec{a, (int32_t)(pn - lo - 1), pa + lo + 1};
+        else
+            atomicOr(bad, 1);  // rows are not symmetric
+    }
+}
+
+// First hub rank: degrees ascend with rank, so the ranks of degree at least
+// min_deg are [sc[1], nv).  One thread.
+__global__ void k_hub_lo(const VInfo *__restrict__ vi, int32_t nv, int32_t min_deg,
+                         unsigned long long *__restrict__ sc)
+{
+    int32_t lo = 0, hi = nv;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (vi[mid].len < min_deg) lo = mid + 1; else hi = mid;
+    }
+    sc[1] = (unsigned long long)lo;
+    sc[2] = 0ull;
+}
+
+// Every N- part below the hubs must have been filled exactly, and the plus
+// entries that point to hubs must be as many as the hubs' tasks (each of
+// which k_orient_transpose_hubs matched to a distinct plus entry); both hold
+// for symmetric rows.
+__global__ void k_check_cursor(const int32_t *__restrict__ cursor, const VInfo *__restrict__ vi,
+                               const int64_t *__restrict__ toff, int32_t nv,
+                               const unsigned long long *__restrict__ sc, int32_t *__restrict__ bad)
+{
+    const int32_t hub_lo = (int32_t)sc[1];
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < hub_lo;
          r += (int64_t)gridDim.x * blockDim.x)
         if (cursor[r] != vi[r].dm) atomicOr(bad, 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0 &&
+        sc[2] != (unsigned long long)(toff[nv] - toff[hub_lo]))
+        atomicOr(bad, 1);
 }
 
 __global__ void k_direct_vinfo(const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
@@ -663,17 +765,10 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
         if (e < nnz && tail && x) atomicAdd(&work[b], (unsigned long long)x);
     }
     if (bstats) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            unsigned long long t = wave_sum<unsigned long long>(st_t[q]);
-            unsigned long long by = wave_sum<unsigned long long>(st_b[q]);
-            unsigned long long it = wave_sum<unsigned long long>(st_i[q]);
-            if (lane == 0 && t) {
-                atomicAdd(&bstats[2 * q], t);
-                atomicAdd(&bstats[2 * q + 1], by);
-                atomicAdd(&bstats[6 + q], it);
-            }
-        }
+        // bstats layout: tasks/bytes pairs per bin, then items per bin
+        const unsigned long long v[9] = {st_t[0], st_b[0], st_t[1], st_b[1], st_t[2],
+                                         st_b[2], st_i[0], st_i[1], st_i[2]};
+        block_add_u64<9>(v, bstats);
     }
 }
 
@@ -717,13 +812,7 @@ __global__ void k_plan_jobs(const VInfo *__restrict__ vi, const unsigned long lo
         njobs[(int64_t)nv + v] = nj[1];
         njobs[2 * (int64_t)nv + v] = nj[2];
     }
-    if (jstats) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const unsigned long long t = wave_sum<unsigned long long>(tile_items[q]);
-            if (lane == 0 && t) atomicAdd(&jstats[q], t);
-        }
-    }
+    if (jstats) block_add_u64<3>(tile_items, jstats);
 }
 
 template <bool ORIENTED>
@@ -1281,6 +1370,9 @@ tc_schedule default_schedule()
 namespace {
 
 constexpr int kL0 = 64, kL1 = 512, kL2 = 4096;
+// owners of at least this degree get their task lists from their own rows
+// (k_orient_transpose_hubs) instead of by appends
+constexpr int kHubDegree = 65536;
 constexpr int kNW0 = 1, kNW1 = 2, kNW2 = 8;
 
 int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const CountArgs &A,
@@ -1326,7 +1418,7 @@ int build_direct_vinfo(Graph &g)
     return TC_OK;
 }
 
-int prepare_views(Graph &g, bool oriented)
+int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
 {
     const int32_t nv = g.st.nv;
     const int64_t nnz = g.st.nnz;
@@ -1377,11 +1469,20 @@ int prepare_views(Graph &g, bool oriented)
     int32_t *order = g.order.as<int32_t>();
     int32_t *rnk = g.rnk.as<int32_t>();
     uint64_t *keys = g.ov[3].as<uint64_t>();
-    hipLaunchKernelGGL(k_rank_keys, dim3(vblocks), dim3(256), 0, s, g.st.len.as<int32_t>(), nv, keys,
-                       order);
+    // device scalars: [0] maximum degree, [1] first hub rank, [2] plus entries
+    // that point to a hub
+    TC_TRY(g.ov[12].ensure(4 * sizeof(unsigned long long)));
+    unsigned long long *sc = g.ov[12].as<unsigned long long>();
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, sc, (int64_t)4);
+    hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(nv, 256, 1024)), dim3(256), 0, s,
+                       g.st.len.as<int32_t>(), nv, keys, order, sc);
     TC_HIP(hipGetLastError());
-    // all 32 bits: the cached maximum degree may predate batch updates
-    TC_TRY(radix_sort_u64(keys, order, nv, 32, g.ov[4], g.ov[5], g.tmp, s));
+    // sort only the bits the current maximum degree needs (one pass per 8)
+    unsigned long long hmax = 0;
+    TC_HIP(hipMemcpyAsync(&hmax, sc, sizeof(hmax), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    const int kbits = std::max(1, ceil_log2_i((int64_t)hmax + 1));
+    TC_TRY(radix_sort_u64(keys, order, nv, kbits, g.ov[4], g.ov[5], g.tmp, s));
     int32_t *dpr = reinterpret_cast<int32_t *>(keys);  // keys are no longer needed
     int32_t *dmr = dpr + nv;
     hipLaunchKernelGGL(k_rank_of, dim3(vblocks), dim3(256), 0, s, order, g.st.len.as<int32_t>(), dmo,
@@ -1398,6 +1499,8 @@ int prepare_views(Graph &g, bool oriented)
     TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
     VInfo *vi = g.vinfo.as<VInfo>();
     hipLaunchKernelGGL(k_rank_vinfo, dim3(vblocks), dim3(256), 0, s, pcoff, dpr, dmr, nv, vi);
+    hipLaunchKernelGGL(k_hub_lo, dim3(1), dim3(1), 0, s, vi, nv,
+                       hub_degree > 0 ? hub_degree : kHubDegree, sc);
     TC_HIP(hipGetLastError());
     int64_t tot[2] = {0, 0};
     TC_HIP(hipMemcpyAsync(&tot[0], pcoff + nv, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -1481,11 +1584,15 @@ int prepare_views(Graph &g, bool oriented)
     int32_t *bad = lcnt + 5;
     hipLaunchKernelGGL(k_fill_u32, dim3(vblocks), dim3(256), 0, s, reinterpret_cast<uint32_t *>(cursor),
                        (int64_t)nv, 0u);
-    if (nplus > 0)
-        hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 1 << 16)), dim3(256), 0, s,
+    if (nplus > 0) {
+        hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 8192)), dim3(256), 0, s,
                            pcoff, pcrow, padj, vi, toff, nv, nplus, g.tasks.as<TaskRec>(), cursor,
-                           bad);
-    hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, nv, bad);
+                           sc, bad);
+        hipLaunchKernelGGL(k_orient_transpose_hubs, dim3(cus * 16), dim3(256), 0, s, pcoff, toff,
+                           order, rnk, ocoff, g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), mask,
+                           cscan, padj, nv, sc, g.tasks.as<TaskRec>(), bad);
+    }
+    hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, toff, nv, sc, bad);
     TC_HIP(hipGetLastError());
     int32_t hbad = 0;
     TC_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1554,7 +1661,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     // compact offsets, chunk rows and the (degree-partitioned) vertex views;
     // rows declared symmetric that turn out not to be go the direct way
     if (oriented) {
-        const int pv = prepare_views(g, true);
+        const int pv = prepare_views(g, true, sched.hub_degree);
         if (pv == TC_ERR_INVALID && sched.method == TC_METHOD_AUTO && !g.symmetric) {
             oriented = false;
             method = TC_METHOD_DIRECT;
@@ -1572,7 +1679,9 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     // planning walks the task array (oriented) or the store rows (direct)
     const int64_t nent = oriented ? g.ntasks : nnz;
     const int64_t nchunks = ceil_div(nent, kWave);
-    const int wblocks = grid_for(nchunks, 4, 1 << 16);  // 4 waves of 256 threads
+    // 4 waves of 256 threads; 16 blocks per CU keep the chip busy and bound
+    // the number of per-block statistics atomics
+    const int wblocks = grid_for(nchunks, 4, 4096);
 
     // ---- planning: work per owner, jobs per bin
     PlanParams P;
@@ -1600,10 +1709,10 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     int32_t *njobs = g.tmp3.as<int32_t>();
     int64_t *jscan = reinterpret_cast<int64_t *>(njobs + 3 * (int64_t)nv + ((3 * (int64_t)nv) & 1));
     if (oriented)
-        hipLaunchKernelGGL(k_plan_jobs<true>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
+        hipLaunchKernelGGL(k_plan_jobs<true>, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, vi,
                            work, nv, P, njobs, stats ? acc + 13 : nullptr);
     else
-        hipLaunchKernelGGL(k_plan_jobs<false>, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, vi,
+        hipLaunchKernelGGL(k_plan_jobs<false>, dim3(grid_for(nv, 256, 4096)), dim3(256), 0, s, vi,
                            work, nv, P, njobs, stats ? acc + 13 : nullptr);
     TC_HIP(hipGetLastError());
     for (int b = 0; b < 3; ++b)

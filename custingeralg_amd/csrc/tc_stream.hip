@@ -148,12 +148,25 @@ __global__ void k_queue_endpoints(const int32_t *__restrict__ pa, const int32_t 
                                   int32_t *__restrict__ mark, int32_t *__restrict__ queue,
                                   unsigned long long *__restrict__ qlen)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        if (!sel[i]) continue;
-        int32_t e[2] = {pa[i], pb[i]};
-        for (int q = 0; q < 2; ++q)
-            if (atomicExch(&mark[e[q]], 1) == 0) queue[atomicAdd(qlen, 1ull)] = e[q];
+    // one queue reservation per wave: appends to a single counter serialise
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); i0 < n;
+         i0 += stride) {
+        const int64_t i = i0 + lane;
+        const bool act = i < n && sel[i];
+        const int32_t e[2] = {act ? pa[i] : 0, act ? pb[i] : 0};
+        for (int q = 0; q < 2; ++q) {
+            const bool won = act && atomicExch(&mark[e[q]], 1) == 0;
+            const uint64_t m = __ballot(won);
+            if (!m) continue;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(qlen, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader, kWave);
+            if (won) queue[base + __popcll(m & lt)] = e[q];
+        }
     }
 }
 
@@ -574,8 +587,8 @@ __global__ __launch_bounds__(kThreads) void k_delta_lb(
         const bool tail = lane == kWave - 1 || tn != t || i + 1 > ilast;
         if (valid && tail && x) atomicAdd(&ecnt[t], x);
     }
-    for (int dd = 32; dd > 0; dd >>= 1) local += __shfl_xor(local, dd);
-    if (lane == 0 && local) atomicAdd(gsum, (unsigned long long)local);
+    const unsigned long long part[1] = {(unsigned long long)local};
+    block_add_u64<1>(part, gsum);  // two's complement: a negative sum wraps
 }
 
 // Per edge: both endpoints get the edge's count.
@@ -593,8 +606,8 @@ __global__ void k_delta_edges(const int32_t *__restrict__ pa, const int32_t *__r
             local += 2 * (int64_t)sign * c;
         }
     }
-    for (int dd = 32; dd > 0; dd >>= 1) local += __shfl_xor(local, dd);
-    if ((threadIdx.x & (kWave - 1)) == 0 && local) atomicAdd(gsum, (unsigned long long)local);
+    const unsigned long long part[1] = {(unsigned long long)local};
+    block_add_u64<1>(part, gsum);
 }
 
 // The reference's three passes at one lane per directed batch entry
@@ -778,10 +791,10 @@ int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel
     if (total == 0) return TC_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, ecnt, n);
     const int64_t nchunks = (total + kWave - 1) / kWave;
-    hipLaunchKernelGGL(k_delta_lb, dim3(grid_for(nchunks, kThreads / kWave, 1 << 16)), dim3(kThreads),
+    hipLaunchKernelGGL(k_delta_lb, dim3(grid_for(nchunks, kThreads / kWave, 4096)), dim3(kThreads),
                        0, s, pa, pb, pre, n, total, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
                        g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), sign, out, ecnt, gsum);
-    hipLaunchKernelGGL(k_delta_edges, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, n, ecnt,
+    hipLaunchKernelGGL(k_delta_edges, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, pa, pb, n, ecnt,
                        sign, out, gsum);
     TC_HIP(hipGetLastError());
     return TC_OK;

@@ -118,7 +118,9 @@ typedef struct {
     int32_t persistent;    /* 1: persistent job-queue launch                          */
     int32_t flush_items;   /* min items per owner slot in one job (bounds the per-job */
                            /*   flush of third-vertex counters)                      */
-    int32_t reserved[8];
+    int32_t hub_degree;    /* oriented: owners of at least this degree build their   */
+                           /*   task lists from their own rows (0: default 65536)    */
+    int32_t reserved[7];
 } tc_schedule;
 
 /* Fill *s with the defaults (autotuned table if one was loaded). */

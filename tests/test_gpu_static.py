@@ -35,6 +35,16 @@ def test_closed_forms(name, policy):
     assert st["method_used"] == T.METHOD_ORIENTED
 
 
+@pytest.mark.parametrize("name", sorted(GR.closed_form_cases()))
+def test_closed_forms_task_lists_from_rows(name):
+    """Every owner's task list built from its own row (hub degree 1)."""
+    off, adj, t = GR.closed_form_cases()[name]
+    pv, glob, st = count(off, adj, hub_degree=1)
+    assert np.array_equal(pv, (2 * t).astype(np.uint32)), name
+    assert glob == 2 * int(t.sum())
+    assert st["method_used"] == T.METHOD_ORIENTED
+
+
 @pytest.mark.parametrize("name", sorted(GR.fixture_cases()))
 @pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
 def test_fixtures_networkx(name, method):
@@ -160,3 +170,64 @@ def test_validation_rejects_malformed():
     adj = np.array([1, 7, 0], np.int32)  # out of range
     with pytest.raises(T.TCError):
         T.cuStinger(off, adj)
+
+
+def _hub_graph(n, hubs, hub_deg, seed):
+    """Sparse R-MAT background plus a few hubs of hub_deg neighbours each (hubs
+    adjacent to each other), so some owners pass the hub degree of the
+    oriented task construction."""
+    rng = np.random.default_rng(seed)
+    u, v = O.rmat_pairs(14, 60000, seed)
+    hu = np.repeat(np.arange(hubs, dtype=np.int32), hub_deg)
+    hv = np.concatenate([rng.choice(np.arange(n, dtype=np.int32), hub_deg, replace=False)
+                         for _ in range(hubs)])
+    return O.canon_csr(n, np.concatenate([u % n, hu]).astype(np.int32),
+                       np.concatenate([v % n, hv]).astype(np.int32))
+
+
+@pytest.mark.parametrize("hub_degree", [0, 1, 3, 16384])
+@pytest.mark.parametrize("policy", POLICIES)
+def test_hub_owners(policy, hub_degree):
+    """Task lists built by appends and from the owners' own rows (every owner
+    from degree 1, from degree 3, only the four hubs of degree 20000+, none at
+    the default) give the same bit-exact counts."""
+    off, adj = _hub_graph(40000, 4, 20000, 3)
+    assert np.sort(np.diff(off))[-4] >= 16384
+    want = O.custinger_count(off, adj)
+    pv, glob, st = count(off, adj, policy=policy, hub_degree=hub_degree)
+    assert st["method_used"] == T.METHOD_ORIENTED
+    assert np.array_equal(pv, want)
+    assert glob == int(want.astype(np.uint64).sum())
+
+
+@pytest.mark.parametrize("hub_degree", [0, 1, 16384])
+@pytest.mark.parametrize("drop", ["hub_side", "leaf_side"])
+def test_declared_symmetric_hub_rows_that_are_not(drop, hub_degree):
+    """Rows declared symmetric (check skipped) with one entry of a hub pair
+    missing: the task construction notices (appends or hub rows) and AUTO
+    takes the direct method, whose result is the reference formula on the
+    rows as given."""
+    off, adj = _hub_graph(40000, 2, 20000, 4)
+    hub = int(np.argmax(np.diff(off)))
+    row = adj[off[hub]:off[hub + 1]]
+    leaf = int(row[len(row) // 2])
+    if drop == "hub_side":
+        r, x = hub, leaf
+    else:
+        r, x = leaf, hub
+    lst = adj[off[r]:off[r + 1]]
+    k = off[r] + int(np.searchsorted(lst, x))
+    adj2 = np.delete(adj, k)
+    off2 = off.copy()
+    off2[r + 1:] -= 1
+    want = O.custinger_count(off2, adj2)
+    g = T.cuStinger(off2, adj2, assume_symmetric=True)
+    try:
+        pv, glob, st = T.static_count(g, T.default_schedule(hub_degree=hub_degree))
+        assert st["method_used"] == T.METHOD_DIRECT
+        assert np.array_equal(pv, want)
+        assert glob == int(want.astype(np.uint64).sum())
+        with pytest.raises(T.TCError):
+            T.static_count(g, T.default_schedule(method=T.METHOD_ORIENTED, hub_degree=hub_degree))
+    finally:
+        g.close()
