@@ -71,7 +71,9 @@ class Stats(C.Structure):
                 ("tasks_bin", C.c_int64 * 4), ("jobs_bin", C.c_int64 * 4),
                 ("strategy_tasks", C.c_int64 * 3), ("triangles", C.c_uint64),
                 ("method_used", C.c_int32), ("reserved0", C.c_int32),
-                ("items_bin", C.c_double * 3), ("stream_bytes_bin", C.c_double * 3)]
+                ("items_bin", C.c_double * 3), ("stream_bytes_bin", C.c_double * 3),
+                ("ms_long", C.c_double * 2), ("tasks_long", C.c_int64 * 2),
+                ("items_long", C.c_double * 2), ("stream_bytes_long", C.c_double * 2)]
 
     def as_dict(self) -> dict:
         return {
@@ -80,6 +82,8 @@ class Stats(C.Structure):
             "tasks_bin": list(self.tasks_bin)[:3], "jobs_bin": list(self.jobs_bin)[:3],
             "items_bin": list(self.items_bin),
             "stream_bytes_bin": list(self.stream_bytes_bin),
+            "ms_long": list(self.ms_long), "tasks_long": list(self.tasks_long),
+            "items_long": list(self.items_long), "stream_bytes_long": list(self.stream_bytes_long),
             "strategy_tasks": dict(zip(("merge", "bsearch", "hash"), self.strategy_tasks)),
             "triangles": int(self.triangles), "method_used": int(self.method_used),
         }

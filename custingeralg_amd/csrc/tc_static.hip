@@ -910,6 +910,11 @@ struct CountArgs {
     // hits are added to per_graph[g] of its owner (triangles never span graphs)
     const int32_t *gid;  // graph of each owner (rank or vertex id, as jobs name them)
     unsigned long long *per_graph;
+    // largest bin, oriented: hash tasks are left to k_count_long, which adds
+    // its tasks, streamed entries and owner-tile entries to long_stats[0..2]
+    // (bitmap tiles) or [3..5] (hash-table tiles)
+    int32_t long_external;
+    unsigned long long *long_stats;
 };
 
 __device__ inline int lg_bucket(int32_t x)
@@ -939,6 +944,26 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
     // searching the owner list into a much longer task list beats streaming it
     if ((int64_t)L * lg_bucket(pl) * 4 < (int64_t)pl) return S_BSEARCH;
     return S_HASH;
+}
+
+// The part of a task list [pb, pb+pl) inside an owner tile's id range
+// [lo_v, hi_v] (tiled owners): two binary searches.
+__device__ inline void tile_filter(const int32_t *__restrict__ adj, int64_t &pb, int32_t &pl,
+                                   int32_t lo_v, int32_t hi_v)
+{
+    int32_t lo = 0, hi = pl;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (adj[pb + mid] < lo_v) lo = mid + 1; else hi = mid;
+    }
+    const int32_t a0 = lo;
+    hi = pl;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (adj[pb + mid] <= hi_v) lo = mid + 1; else hi = mid;
+    }
+    pb += a0;
+    pl = lo - a0;
 }
 
 template <int LMAX, int NW, bool ORIENTED>
@@ -991,6 +1016,19 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
         const bool use_bm = kUseBitmap && nwords <= kBmWords;
         uint2 *BMw = reinterpret_cast<uint2 *>(HB);
         const TileBitmap BT{BMw, lo_v, span};
+        if (ORIENTED && !kChunkedLong && A.long_external) {
+            // k_count_long takes every hash task: only a job holding a task of
+            // another strategy needs its tile here
+            bool other = false;
+            for (int32_t t = J.t_begin + threadIdx.x; t < J.t_end; t += nthr) {
+                const TaskRec tk = A.tasks[tbase + t];
+                int32_t pl = tk.len;
+                int64_t pb = tk.base;
+                if (tiled && pl > 0) tile_filter(A.adj, pb, pl, lo_v, hi_v);
+                if (pl > 0 && choose_strategy(L, pl, A) != S_HASH) other = true;
+            }
+            if (!__syncthreads_or(other)) continue;
+        }
         if (use_bm) {
             for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
             if (ORIENTED)
@@ -1087,7 +1125,12 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
                 pb += a0;
                 pl = lo - a0;
             }
-            const int strat = (pl > 0) ? choose_strategy(L, pl, A) : -1;
+            int strat = (pl > 0) ? choose_strategy(L, pl, A) : -1;
+            if (ORIENTED && !kChunkedLong && A.long_external && strat == S_HASH) {
+                strat = -1;  // k_count_long's task
+                pl = 0;
+                valid = false;
+            }
             if (strat >= 0) sc[strat]++;
             tpb[wid][lane] = pb;
             tpl[wid][lane] = pl;
@@ -1351,6 +1394,197 @@ __global__ __launch_bounds__(NW * kWave) void k_count(const Job *__restrict__ jo
     }
 }
 
+// Hash-strategy tasks of the largest bin, oriented method (almost all of the
+// bin's items).  A wave's 64 tasks are streamed as one sequence of 64-entry
+// chunks (chunk c belongs to the first task whose inclusive chunk count
+// exceeds c; chunk owners are wave-uniform), kLongUnroll chunks per batch, and
+// the next batch's loads are issued before the current batch is probed, so
+// two batches are in flight per wave while the LDS probes run.  BM = true
+// takes the jobs whose owner tile fits the bitmap and needs only the bitmap
+// and L counters in LDS (three blocks per CU); BM = false takes the
+// hash-table tiles.  Each skips the other's jobs; k_count<LMAX, NW, true>
+// with long_external does the tasks of the other strategies.
+constexpr int kLongUnroll = 8;
+
+template <int LMAX, int NW, bool BM>
+__global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job *__restrict__ jobs,
+                                                           int64_t njobs, CountArgs A)
+{
+    constexpr int NBMAX = LMAX / 2;
+    constexpr int kBmWords = 2 * NBMAX;  // uint2 words in HB's space
+    __shared__ int4 HB[NBMAX];
+    __shared__ uint32_t C[BM ? LMAX : 4 * NBMAX];
+    __shared__ unsigned long long red[NW];
+    __shared__ int32_t wsum[NW];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int nthr = NW * kWave;
+    unsigned long long hits_all = 0, n_hash = 0, n_items = 0, n_tile = 0;
+
+    for (int64_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+        const Job J = jobs[jb];
+        const VInfo vo = A.vi[J.owner];
+        const int64_t tbase = A.toff[J.owner];
+        const int32_t Lfull = vo.len - vo.dm;
+        const int32_t s0 = J.tile * LMAX;
+        const int32_t L = min(LMAX, Lfull - s0);
+        const bool tiled = Lfull > LMAX;
+        const int32_t *Sg = A.adj + vo.rb + s0;
+        const int32_t lo_v = Sg[0], hi_v = Sg[L - 1];
+        const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
+        const int32_t nwords = (int32_t)((span + 31u) >> 5);
+        if ((nwords <= kBmWords) != BM) continue;  // the other instantiation's job
+        int nbb = 0;
+        while ((1 << nbb) * 2 < L) ++nbb;
+        const BucketTable T{HB, nbb};
+        uint2 *BMw = reinterpret_cast<uint2 *>(HB);
+        const TileBitmap BT{BMw, lo_v, span};
+        if (BM) {
+            for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
+            for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < L; i += nthr) {
+                const uint32_t off = (uint32_t)(Sg[i] - lo_v);
+                atomicOr(&BMw[off >> 5].x, 1u << (off & 31u));
+            }
+            __syncthreads();
+            const int32_t per = (nwords + nthr - 1) / nthr;
+            const int32_t k0 = threadIdx.x * per, k1 = min(nwords, k0 + per);
+            int32_t mine = 0;
+            for (int32_t k = k0; k < k1; ++k) mine += __popc(BMw[k].x);
+            const int32_t incl = wave_incl_scan_i32(mine);
+            if (lane == kWave - 1) wsum[wid] = incl;
+            __syncthreads();
+            int32_t run = incl - mine;
+            for (int w = 0; w < wid; ++w) run += wsum[w];
+            for (int32_t k = k0; k < k1; ++k) {
+                BMw[k].y = (uint32_t)run;
+                run += __popc(BMw[k].x);
+            }
+            __syncthreads();
+        } else {
+            for (int i = threadIdx.x; i < (1 << nbb); i += nthr) HB[i] = make_int4(-1, -1, -1, -1);
+            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) C[i] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
+            __syncthreads();
+        }
+        auto probe = [&](const int32_t (&w)[kLongUnroll], int32_t (&slot)[kLongUnroll]) {
+            if (BM) {
+                BT.find_n(w, slot);
+            } else {
+#pragma unroll
+                for (int u = 0; u < kLongUnroll; u += 2) T.find_n<2>(&w[u], &slot[u]);
+            }
+        };
+        unsigned long long hits_job = 0;
+
+        // the same round-robin deal of tasks to waves as k_count
+        for (int32_t g0 = 0; J.t_begin + (g0 * kWave) * NW + wid < J.t_end; ++g0) {
+            const int32_t t = J.t_begin + (g0 * kWave + lane) * NW + wid;
+            int32_t pl = 0;
+            int64_t pb = 0, sb = 0;
+            if (t < J.t_end) {
+                const TaskRec tk = A.tasks[tbase + t];
+                pl = tk.len;
+                sb = tk.base;
+                pb = sb;
+            }
+            if (tiled && pl > 0) tile_filter(A.adj, pb, pl, lo_v, hi_v);
+            if (!(pl > 0 && choose_strategy(L, pl, A) == S_HASH)) pl = 0;
+            n_hash += pl > 0;
+            n_items += (unsigned long long)pl;
+            const int32_t myc = (pl + kWave - 1) / kWave;
+            const int32_t cincl = wave_incl_scan_i32(myc);
+            const int32_t ctot = __builtin_amdgcn_readlane(cincl, kWave - 1);
+            if (ctot == 0) continue;
+            const int32_t cexcl = cincl - myc;
+            // chunks c0 .. c0+kLongUnroll-1: their entries and owner lanes
+            auto load = [&](int32_t c0, int32_t (&w)[kLongUnroll], int32_t (&jj)[kLongUnroll]) {
+#pragma unroll
+                for (int u = 0; u < kLongUnroll; ++u) {
+                    const int32_t c = c0 + u;
+                    w[u] = -1;
+                    jj[u] = -1;
+                    if (c < ctot) {
+                        const int j = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
+                        const int32_t cf = __builtin_amdgcn_readlane(cexcl, j);
+                        const int32_t n = __builtin_amdgcn_readlane(pl, j);
+                        const int32_t *lst = A.adj + readlane_i64(pb, j);
+                        const int32_t k = (c - cf) * kWave + lane;
+                        if (k < n) w[u] = lst[k];
+                        jj[u] = j;
+                    }
+                }
+            };
+            uint32_t mine_h = 0;  // hits of this lane's own task
+            int32_t cur[kLongUnroll], cj[kLongUnroll];
+            load(0, cur, cj);
+            for (int32_t c0 = 0;; c0 += kLongUnroll) {
+                const bool more = c0 + kLongUnroll < ctot;
+                int32_t nxt[kLongUnroll], nj[kLongUnroll];
+                if (more) load(c0 + kLongUnroll, nxt, nj);
+                int32_t slots[kLongUnroll];
+                probe(cur, slots);
+#pragma unroll
+                for (int u = 0; u < kLongUnroll; ++u) {
+                    const bool hit = slots[u] >= 0;
+                    if (hit) atomicAdd(&C[slots[u]], 1u);
+                    const uint32_t hc = (uint32_t)__popcll(__ballot(hit));
+                    if (lane == cj[u]) mine_h += hc;
+                }
+                if (!more) break;
+#pragma unroll
+                for (int u = 0; u < kLongUnroll; ++u) {
+                    cur[u] = nxt[u];
+                    cj[u] = nj[u];
+                }
+            }
+            if (mine_h) {
+                hits_job += mine_h;
+                // a tiled owner meets the task once per tile
+                if (tiled)
+                    atomicAdd(&A.pcnt[sb - 1], mine_h);
+                else
+                    A.pcnt[sb - 1] = mine_h;
+            }
+        }
+        // owner + third-vertex flush
+        const unsigned long long hj = wave_sum<unsigned long long>(hits_job);
+        if (lane == 0) red[wid] = hj;
+        __syncthreads();
+        unsigned long long hjob = 0;
+        for (int w = 0; w < NW; ++w) hjob += red[w];
+        if (BM) {
+            for (int i = threadIdx.x; i < L; i += nthr) {
+                const uint32_t ci = C[i];
+                if (ci) atomicAdd(&A.out[A.order[Sg[i]]], 2u * ci);
+            }
+        } else {
+            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
+                const uint32_t ci = C[i];
+                if (ci) atomicAdd(&A.out[A.order[T.id_at(i)]], 2u * ci);
+            }
+        }
+        if (threadIdx.x == 0 && hjob) {
+            atomicAdd(&A.out[A.order[J.owner]], (uint32_t)(2 * hjob));
+            if (A.per_graph) atomicAdd(&A.per_graph[A.gid[J.owner]], 6ull * hjob);
+        }
+        hits_all += (threadIdx.x == 0) ? hjob : 0ull;
+        n_tile += (threadIdx.x == 0) ? (unsigned long long)L : 0ull;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && hits_all) atomicAdd(A.global, 6ull * hits_all);
+    if (A.strat) {
+        const unsigned long long nh = wave_sum<unsigned long long>(n_hash);
+        if (lane == 0 && nh) atomicAdd(&A.strat[S_HASH], nh);
+    }
+    if (A.long_stats) {
+        const unsigned long long v[3] = {n_hash, n_items, n_tile};
+        block_add_u64<3>(v, A.long_stats + (BM ? 0 : 3));
+    }
+}
+
 // ------------------------------------------------------------- driver --
 
 tc_schedule default_schedule()
@@ -1375,8 +1609,10 @@ constexpr int kL0 = 64, kL1 = 512, kL2 = 4096;
 constexpr int kHubDegree = 65536;
 constexpr int kNW0 = 1, kNW1 = 2, kNW2 = 8;
 
+// ev_long (bin 2, oriented): events recorded before and after the bitmap-tile
+// launch of k_count_long and after its hash-table-tile launch.
 int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const CountArgs &A,
-               hipStream_t s)
+               hipStream_t s, hipEvent_t *ev_long)
 {
     if (njobs <= 0) return TC_OK;
     int dev = 0;
@@ -1397,9 +1633,20 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
         else
             hipLaunchKernelGGL((k_count<kL1, kNW1, false>), dim3((unsigned)grid), dim3(kNW1 * kWave), 0, s, jobs, njobs, A);
     } else {
-        if (oriented)
-            hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
-        else
+        if (oriented) {
+            // short tasks here, long hash tasks in k_count_long (bitmap and
+            // hash-table tiles in separate launches for their LDS footprints)
+            CountArgs As = A;
+            As.long_external = 1;
+            hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, As);
+            const int64_t gb = std::min<int64_t>(njobs, (int64_t)cus * 3);
+            const int64_t gh = std::min<int64_t>(njobs, (int64_t)cus * 2);
+            TC_HIP(hipEventRecord(ev_long[0], s));
+            hipLaunchKernelGGL((k_count_long<kL2, kNW2, true>), dim3((unsigned)gb), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
+            TC_HIP(hipEventRecord(ev_long[1], s));
+            hipLaunchKernelGGL((k_count_long<kL2, kNW2, false>), dim3((unsigned)gh), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
+            TC_HIP(hipEventRecord(ev_long[2], s));
+        } else
             hipLaunchKernelGGL((k_count<kL2, kNW2, false>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
     }
     TC_HIP(hipGetLastError());
@@ -1634,8 +1881,9 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     }
     bool oriented = method == TC_METHOD_ORIENTED;
 
-    // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin stats
-    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
+    // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin tasks
+    // and bytes, [10..12] items, [13..15] tile items, [16..21] long-task stats
+    TC_TRY(g.accum.ensure(24 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
     uint32_t *outp = out;
     if (!outp) {
@@ -1644,7 +1892,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     }
 
     TC_HIP(hipEventRecord(g.ev[0], s));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, acc, (int64_t)16);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, acc, (int64_t)24);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, outp,
                        (int64_t)nv, 0u);
     TC_HIP(hipGetLastError());
@@ -1736,7 +1984,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     TC_HIP(hipEventRecord(g.ev[2], s));
 
     // ---- count
-    CountArgs A;
+    CountArgs A = {};
     A.adj = adj;
     A.vi = vi;
     A.tasks = tasks;
@@ -1746,6 +1994,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.out = outp;
     A.global = acc;
     A.strat = acc + 1;
+    A.long_stats = acc + 16;
     A.policy = sched.policy;
     A.ratio_x16 = sched.ratio_x16 > 0 ? sched.ratio_x16 : 64;
     A.skip_lower = skip_lower ? 1 : 0;
@@ -1777,7 +2026,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     const Job *jl[3] = {j0, j1, j2};
     for (int b = 0; b < 3; ++b) {
         TC_HIP(hipEventRecord(g.ev[3 + b], s));
-        TC_TRY(launch_bin(b, oriented, jl[b], jt[b], A, s));
+        TC_TRY(launch_bin(b, oriented, jl[b], jt[b], A, s, &g.ev[8]));
     }
     TC_HIP(hipEventRecord(g.ev[6], s));  // end of the bin kernels
     if (oriented && nplus > 0) {
@@ -1789,7 +2038,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     TC_HIP(hipEventRecord(g.ev[7], s));
     TC_HIP(hipEventSynchronize(g.ev[7]));
 
-    unsigned long long hacc[16];
+    unsigned long long hacc[24];
     TC_HIP(hipMemcpy(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost));
     if (global) *global = hacc[0];
     if (stats) {
@@ -1811,6 +2060,16 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
                                      4.0 * (double)hacc[13 + b];
         }
         for (int q = 0; q < 3; ++q) st.strategy_tasks[q] = (int64_t)hacc[1 + q];
+        if (oriented && jt[2] > 0) {
+            for (int k = 0; k < 2; ++k) {
+                (void)hipEventElapsedTime(&ms, g.ev[8 + k], g.ev[9 + k]);
+                st.ms_long[k] = ms;
+                const unsigned long long *x = hacc + 16 + 3 * k;
+                st.tasks_long[k] = (int64_t)x[0];
+                st.items_long[k] = (double)x[1];
+                st.stream_bytes_long[k] = 4.0 * (double)x[1] + 24.0 * (double)x[0] + 4.0 * (double)x[2];
+            }
+        }
         st.triangles = oriented ? hacc[0] / 6 : 0;
         *stats = st;
     }

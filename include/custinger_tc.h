@@ -142,6 +142,14 @@ typedef struct {
     double stream_bytes_bin[3]; /* bytes each bin must read: 4 per streamed
                                    entry + 24 per task (task entry and vertex
                                    record) + 4 per owner-tile entry per job */
+    /* largest bin, oriented method: its hash-strategy tasks run in a kernel of
+       their own, one launch for bitmap tiles [0] and one for hash-table tiles
+       [1]; their hipEvent times, tasks, streamed entries and bytes (the same
+       unit as stream_bytes_bin) */
+    double ms_long[2];
+    int64_t tasks_long[2];
+    double items_long[2];
+    double stream_bytes_long[2];
 } tc_stats;
 
 /* ------------------------------------------------------ static count -- */
