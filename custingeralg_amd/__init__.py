@@ -49,6 +49,7 @@ EXPORTS = [
     "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune", "tc_schedule_set_default",
     "tc_batched_graph_create", "tc_batched_graph_count", "tc_read_graph", "tc_host_free",
+    "tc_lb_edge_prefix", "tc_graph_get_view",
 ]
 
 

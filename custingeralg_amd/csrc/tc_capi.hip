@@ -25,6 +25,7 @@ int batched_count(int32_t G, const int64_t *voff, const int64_t *off, const int3
                   const tc_schedule &sched, tc_stats *stats, hipStream_t s);
 int lb_partition(Graph &g, const int32_t *verts, int64_t len, int64_t *parts, int64_t *P);
 int lb_degree_bins(Graph &g, const int32_t *verts, int64_t len, int64_t counts[3]);
+int lb_edge_prefix(Graph &g, const int32_t *verts, int64_t len, int64_t *pre);
 int gen_rmat(int32_t scale, int64_t m, double a, double b, double c, uint64_t seed, int32_t *u,
              int32_t *v, hipStream_t s);
 int gen_er(int32_t n, int64_t m, uint64_t seed, int32_t *u, int32_t *v, hipStream_t s);
@@ -366,6 +367,33 @@ int tc_lb_partition(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t 
     TC_GUARD({
         TC_CHECK_ARG(g && verts_dev && len > 0 && parts_dev && P, "tc_lb_partition: bad arguments");
         return tc::lb_partition(*g->g, verts_dev, len, parts_dev, P);
+    })
+}
+
+int tc_lb_edge_prefix(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t *pre_dev)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(g && pre_dev, "tc_lb_edge_prefix: null argument");
+        if (!verts_dev) len = g->g->st.nv;
+        TC_CHECK_ARG(len >= 0, "tc_lb_edge_prefix: bad length");
+        return tc::lb_edge_prefix(*g->g, verts_dev, len, pre_dev);
+    })
+}
+
+int tc_graph_get_view(tc_graph *g, tc_graph_view *view)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(g && view, "tc_graph_get_view: null argument");
+        TC_CHECK_ARG(!g->g->batch_voff, "tc_graph_get_view: not available on a batched handle");
+        const tc::Graph &G = *g->g;
+        std::memset(view, 0, sizeof(*view));
+        view->nv = G.st.nv;
+        view->nnz = G.st.nnz;
+        view->boff = G.st.boff.as<int64_t>();
+        view->len = G.st.len.as<int32_t>();
+        view->pool = G.st.pool.as<int32_t>();
+        view->stream = (void *)G.stream;
+        return TC_OK;
     })
 }
 

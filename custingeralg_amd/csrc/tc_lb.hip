@@ -15,12 +15,13 @@ namespace tc {
 
 namespace {
 
+// work[i] = degree of verts[i] (of vertex i when verts is null), work[len] = 0
 __global__ void k_lb_work(const int32_t *__restrict__ verts, int64_t len,
                           const int32_t *__restrict__ deg, int64_t *__restrict__ work)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= len;
          i += (int64_t)gridDim.x * blockDim.x)
-        work[i] = i < len ? (int64_t)deg[verts[i]] : 0;
+        work[i] = i < len ? (int64_t)deg[verts ? verts[i] : (int32_t)i] : 0;
 }
 
 __global__ void k_lb_parts(const int64_t *__restrict__ prefix, int64_t len, int64_t P,
@@ -98,6 +99,14 @@ int lb_partition(Graph &g, const int32_t *verts, int64_t len, int64_t *parts, in
     }
     *P = np;
     return done(TC_OK);
+}
+
+int lb_edge_prefix(Graph &g, const int32_t *verts, int64_t len, int64_t *pre)
+{
+    hipLaunchKernelGGL(k_lb_work, dim3(grid_for(len + 1, 256, 65536)), dim3(256), 0, g.stream, verts,
+                       len, g.st.len.as<int32_t>(), pre);
+    TC_HIP(hipGetLastError());
+    return scan_exclusive_i64(pre, pre, len + 1, nullptr, g.tmp, g.stream);
 }
 
 int lb_degree_bins(Graph &g, const int32_t *verts, int64_t len, int64_t counts[3])

@@ -247,6 +247,32 @@ int tc_lb_partition(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t 
 /* Degree binning of the LB traversal (include/operators.cuh:164-288):
  * counts of vertices with degree >= 64, in [16,64) and < 16 among verts. */
 int tc_lb_degree_bins(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t counts[3]);
+/* Edge numbering of the load-balanced traversal (allVinA_TraverseEdges_LB,
+ * include/operators.cuh:291-310, on the prefix of cusLoadBalance's work
+ * array, src/load_balance.cu:32-54): pre_dev[i] = sum of the degrees of
+ * verts_dev[0..i) for i in [0, len] (device int64, len+1 entries).  verts_dev
+ * NULL means every vertex (len is then nv).  Runs on the handle's stream. */
+int tc_lb_edge_prefix(tc_graph *g, const int32_t *verts_dev, int64_t len, int64_t *pre_dev);
+
+/* ------------------------------------------------------ operator views -- */
+/* Device view of the store for user kernels: what the reference's operators
+ * read through cuStinger::devicePtr(), dVD->used[v] and dVD->adj[v]->dst
+ * (include/operators.cuh:73-74, 137-140, 226-227).  Vertex v's neighbours are
+ * pool[boff[v] .. boff[v]+len[v]), sorted ascending.  The pointers stay valid
+ * until the next update or destroy; stream is the handle's stream, on which
+ * kernels over the view are ordered after the library's own work.  Not
+ * available on a batched handle.  include/custinger_ops.hpp builds the
+ * traversal operators on it. */
+typedef struct {
+    int32_t nv;
+    int32_t reserved;
+    int64_t nnz;
+    const int64_t *boff;
+    const int32_t *len;
+    const int32_t *pool;
+    void *stream;
+} tc_graph_view;
+int tc_graph_get_view(tc_graph *g, tc_graph_view *view);
 
 /* --------------------------------------------------------- graph files -- */
 /* readGraphDIMACS / readGraphSNAP / readGraphMatrixMarket, the external

@@ -87,6 +87,18 @@ def main():
         "csr_sha256": hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest(),
         "pv_sha256": hashlib.sha256(pv.tobytes()).hexdigest(),
     }
+    # R-MAT scale 20, ef 16, seed 2: the same hashes at the scale-20 config;
+    # the per-vertex vector is the forward count's, checked here against the
+    # reference-formula restatement (about a minute)
+    off, adj = O.rmat_graph(20, 16, 2)
+    T, tv = O.forward_count(off, adj, per_vertex=True)
+    pv = (2 * tv).astype(np.uint32)
+    assert np.array_equal(pv, O.custinger_count(off, adj))
+    summary["rmat20"] = {
+        "nnz": int(len(adj)), "T": int(T),
+        "csr_sha256": hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest(),
+        "pv_sha256": hashlib.sha256(pv.tobytes()).hexdigest(),
+    }
     np.savez_compressed(os.path.join(HERE, "fixtures.npz"), **out)
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(summary, f, indent=1, sort_keys=True)

@@ -231,3 +231,20 @@ def test_declared_symmetric_hub_rows_that_are_not(drop, hub_degree):
             T.static_count(g, T.default_schedule(method=T.METHOD_ORIENTED, hub_degree=hub_degree))
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
+def test_rmat20_golden_hashes(method):
+    """Scale-20 config from the device generator: its CSR and the per-vertex
+    counts hash to the committed fixture values (bit-exact at 31.4M entries)."""
+    import hashlib
+    meta = json.load(open(os.path.join(GR.GOLDEN, "fixtures.json")))["rmat20"]
+    g = T.rmat_graph_device(20, 16, 2)
+    try:
+        off, adj = g.to_csr()
+        assert hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest() == meta["csr_sha256"]
+        pv, glob, st = T.static_count(g, T.default_schedule(method=method))
+        assert hashlib.sha256(pv.tobytes()).hexdigest() == meta["pv_sha256"]
+        assert glob == 6 * meta["T"]
+    finally:
+        g.close()

@@ -64,6 +64,21 @@ def test_fixture_hashes_reproduce():
     assert hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest() == meta["rmat16"]["csr_sha256"]
 
 
+def test_rmat20_golden_hashes():
+    """Scale-20 config: the canonical CSR and the per-vertex vector (forward
+    count, 2t(v)) hash to the committed values, which the fixture script
+    checked against the reference-formula restatement."""
+    import hashlib
+    meta = json.load(open(os.path.join(GR.GOLDEN, "fixtures.json")))["rmat20"]
+    off, adj = O.rmat_graph(20, 16, 2)
+    assert len(adj) == meta["nnz"]
+    assert hashlib.sha256(off.tobytes() + adj.tobytes()).hexdigest() == meta["csr_sha256"]
+    T, tv = O.forward_count(off, adj, per_vertex=True)
+    assert T == meta["T"]
+    pv = (2 * tv).astype(np.uint32)
+    assert hashlib.sha256(pv.tobytes()).hexdigest() == meta["pv_sha256"]
+
+
 # --------------------------------------------------------- count formulas
 @pytest.mark.parametrize("name", sorted(GR.closed_form_cases()))
 def test_closed_forms(name):
