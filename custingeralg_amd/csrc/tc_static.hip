@@ -938,9 +938,12 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
         break;
     default: break;
     }
-    // fixed degree-ratio heuristic
+    // fixed degree-ratio heuristic; merge only when ratio_x16 asks for it:
+    // measured on 4096 batched R-MAT graphs, merging the short similar lists
+    // it used to take (long <= 4 * short, long <= 96) cost 1.49x against
+    // streaming them through the table (lane-sequential walks in lockstep)
     int32_t sh = min(L, pl), lo = max(L, pl);
-    if ((int64_t)lo * 16 <= (int64_t)sh * A.ratio_x16 && lo <= 96) return S_MERGE;
+    if (A.ratio_x16 > 0 && (int64_t)lo * 16 <= (int64_t)sh * A.ratio_x16 && lo <= 96) return S_MERGE;
     // searching the owner list into a much longer task list beats streaming it
     if ((int64_t)L * lg_bucket(pl) * 4 < (int64_t)pl) return S_BSEARCH;
     return S_HASH;
@@ -1599,7 +1602,7 @@ tc_schedule default_schedule()
     s.small_max = 64;
     s.mid_max = 512;
     s.job_items = 8192;
-    s.ratio_x16 = 64;  // merge when long <= 4x short (and short lists)
+    s.ratio_x16 = 0;   // no merges by the fixed heuristic (64: long <= 4x short, <= 96)
     s.persistent = 1;
     s.flush_items = 1024;
     return s;
@@ -2000,7 +2003,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.strat = acc + 1;
     A.long_stats = acc + 16;
     A.policy = sched.policy;
-    A.ratio_x16 = sched.ratio_x16 > 0 ? sched.ratio_x16 : 64;
+    A.ratio_x16 = std::max(sched.ratio_x16, 0);
     A.skip_lower = skip_lower ? 1 : 0;
     A.per_graph = g.batch_per_graph;
     A.gid = nullptr;

@@ -113,8 +113,8 @@ typedef struct {
     int32_t small_max;     /* rows with |N+| <= small_max use one wave per job      */
     int32_t mid_max;       /* rows with |N+| <= mid_max use the mid bin              */
     int32_t job_items;     /* target items per job (prefix-sum partition grain)      */
-    int32_t ratio_x16;     /* fixed heuristic: bsearch when short*log2(long)*16 <    */
-                           /*   long*ratio_x16                                        */
+    int32_t ratio_x16;     /* fixed heuristic: merge when long*16 <= short*ratio_x16 */
+                           /*   and long <= 96 (0, the default: never merge)         */
     int32_t persistent;    /* 1: persistent job-queue launch                          */
     int32_t flush_items;   /* min items per owner slot in one job (bounds the per-job */
                            /*   flush of third-vertex counters)                      */

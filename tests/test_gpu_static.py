@@ -104,7 +104,8 @@ def test_rmat18_schedules_identical():
     want = (2 * tv % (1 << 32)).astype(np.uint32)
     variants = [dict(), dict(small_max=16, mid_max=128, job_items=1024),
                 dict(small_max=64, mid_max=512, job_items=65536), dict(policy=T.POLICY_HASH),
-                dict(policy=T.POLICY_BSEARCH), dict(policy=T.POLICY_MERGE)]
+                dict(policy=T.POLICY_BSEARCH), dict(policy=T.POLICY_MERGE),
+                dict(ratio_x16=64)]  # the fixed heuristic with its merges on
     for v in variants:
         pv, glob, _ = count(off, adj, **v)
         assert np.array_equal(pv, want), v
