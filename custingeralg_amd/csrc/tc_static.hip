@@ -338,6 +338,24 @@ __global__ void k_rank_keys(const int32_t *__restrict__ len, int32_t nv, uint64_
     }
 }
 
+// Batched graphs: keys |= graph of the vertex << dbits, so ranks run graph by
+// graph ((graph, degree, id) order).  Every edge lies inside one graph, where
+// the order is still (degree, id): the orientation is unchanged, and each
+// graph's plus lists and task records become contiguous.
+__global__ void k_rank_keys_graph(const int64_t *__restrict__ voff, int32_t G, int32_t nv, int dbits,
+                                  uint64_t *__restrict__ keys)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (voff[mid] <= v) lo = mid; else hi = mid - 1;
+        }
+        keys[v] |= (uint64_t)lo << dbits;
+    }
+}
+
 // rnk[order[r]] = r, and |N+| / |N-| of the vertex of rank r.
 __global__ void k_rank_of(const int32_t *__restrict__ order, const int32_t *__restrict__ len,
                           const int32_t *__restrict__ dmo, int32_t nv, int32_t *__restrict__ rnk,
@@ -634,7 +652,10 @@ __global__ void k_orient_transpose_hubs(
 }
 
 // First hub rank: degrees ascend with rank, so the ranks of degree at least
-// min_deg are [sc[1], nv).  One thread.
+// min_deg are [sc[1], nv).  (With graph-major ranks of batched graphs they
+// ascend per graph only; the search then lands on some rank, and owners above
+// it take the row-built task lists, which are correct for any owner.)  One
+// thread.
 __global__ void k_hub_lo(const VInfo *__restrict__ vi, int32_t nv, int32_t min_deg,
                          unsigned long long *__restrict__ sc)
 {
@@ -1735,7 +1756,13 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     unsigned long long hmax = 0;
     TC_HIP(hipMemcpyAsync(&hmax, sc, sizeof(hmax), hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
-    const int kbits = std::max(1, ceil_log2_i((int64_t)hmax + 1));
+    int kbits = std::max(1, ceil_log2_i((int64_t)hmax + 1));
+    if (g.batch_voff && g.batch_ngraphs > 1) {
+        hipLaunchKernelGGL(k_rank_keys_graph, dim3(vblocks), dim3(256), 0, s, g.batch_voff,
+                           g.batch_ngraphs, nv, kbits, keys);
+        TC_HIP(hipGetLastError());
+        kbits += ceil_log2_i((int64_t)g.batch_ngraphs);
+    }
     TC_TRY(radix_sort_u64(keys, order, nv, kbits, g.ov[4], g.ov[5], g.tmp, s));
     int32_t *dpr = reinterpret_cast<int32_t *>(keys);  // keys are no longer needed
     int32_t *dmr = dpr + nv;
