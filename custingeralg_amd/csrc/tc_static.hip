@@ -936,6 +936,11 @@ struct CountArgs {
     // (bitmap tiles) or [3..5] (hash-table tiles)
     int32_t long_external;
     unsigned long long *long_stats;
+    // largest bin, oriented (every kernel of it): the learned table is not
+    // consulted and the fixed heuristic decides -- a task of a strategy other
+    // than hash makes k_count rebuild its job's tile, a per-job cost the
+    // per-task timings behind the table never see
+    int32_t no_learned_table;
 };
 
 __device__ inline int lg_bucket(int32_t x)
@@ -951,7 +956,7 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
     case TC_POLICY_BSEARCH: return S_BSEARCH;
     case TC_POLICY_HASH: return S_HASH;
     case TC_POLICY_LEARNED:
-        if (A.table) {
+        if (A.table && !A.no_learned_table) {
             int i = min(lg_bucket(L), A.table_dim - 1);
             int j = min(lg_bucket(pl), A.table_dim - 1);
             return A.table[i * A.table_dim + j];
@@ -1664,15 +1669,18 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
         if (oriented) {
             // short tasks here, long hash tasks in k_count_long (bitmap and
             // hash-table tiles in separate launches for their LDS footprints)
-            CountArgs As = A;
+            // (the strategy rules must be the same in all three launches)
+            CountArgs Al = A;
+            Al.no_learned_table = 1;
+            CountArgs As = Al;
             As.long_external = 1;
             hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, As);
             const int64_t gb = std::min<int64_t>(njobs, (int64_t)cus * 3);
             const int64_t gh = std::min<int64_t>(njobs, (int64_t)cus * 2);
             TC_HIP(hipEventRecord(ev_long[0], s));
-            hipLaunchKernelGGL((k_count_long<kL2, kNW2, true>), dim3((unsigned)gb), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count_long<kL2, kNW2, true>), dim3((unsigned)gb), dim3(kNW2 * kWave), 0, s, jobs, njobs, Al);
             TC_HIP(hipEventRecord(ev_long[1], s));
-            hipLaunchKernelGGL((k_count_long<kL2, kNW2, false>), dim3((unsigned)gh), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
+            hipLaunchKernelGGL((k_count_long<kL2, kNW2, false>), dim3((unsigned)gh), dim3(kNW2 * kWave), 0, s, jobs, njobs, Al);
             TC_HIP(hipEventRecord(ev_long[2], s));
         } else
             hipLaunchKernelGGL((k_count<kL2, kNW2, false>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
