@@ -50,6 +50,8 @@ def test_schedule_default_without_gpu():
     s = T.default_schedule()
     assert s.method == T.METHOD_AUTO and s.policy == T.POLICY_FIXED
     assert 1 <= s.small_max <= s.mid_max and s.job_items > 0
+    # no merges by the fixed heuristic, built-in hub degree
+    assert s.ratio_x16 == 0 and s.hub_degree == 0
 
 
 def test_host_validation_rejects_bad_csr():
