@@ -21,10 +21,14 @@ int DevBuf::ensure(size_t need)
 {
     if (need == 0) need = 16;
     if (bytes >= need) return TC_OK;
+    // a buffer that has to grow again gets 25% headroom: batch-sized scratch
+    // varies from call to call, and every regrowth stalls the device (free,
+    // then allocate); a first allocation is exact
     if (p) {
         (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        need += need / 4;
     }
     hipError_t e = hipMalloc(&p, need);
     if (e != hipSuccess) {
