@@ -532,6 +532,9 @@ __device__ inline int64_t edge_of(const int64_t *__restrict__ pre, int64_t i, in
     return lo;
 }
 
+// Each wave takes two chunks of 64 items per pass (c and c + the grid's wave
+// count) and runs their two binary searches interleaved, so every lane keeps
+// two independent load chains in flight.
 __global__ __launch_bounds__(kThreads) void k_delta_lb(
     const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
     const int64_t *__restrict__ pre, int64_t n, int64_t total,
@@ -543,49 +546,83 @@ __global__ __launch_bounds__(kThreads) void k_delta_lb(
     const int64_t nchunks = (total + kWave - 1) / kWave;
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     int64_t local = 0;
-    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
-         c += wstride) {
+    // one item: its edge, the search side and the key
+    struct Item {
+        bool valid, s_short;
+        int64_t i, ilast, t, sb, lb, k;
+        int32_t ll, x;
+    };
+    auto setup = [&](int64_t c, Item &it) {
         const int64_t i0 = c * kWave;
-        const int64_t ilast = min(i0 + kWave, total) - 1;
+        it.ilast = min(i0 + kWave, total) - 1;
         int64_t tb = 0;
         if (lane == 0) tb = edge_of(pre, i0, 0, n - 1);
-        if (lane == 1) tb = edge_of(pre, ilast, 0, n - 1);
+        if (lane == 1) tb = edge_of(pre, it.ilast, 0, n - 1);
         const int64_t tfirst = __shfl(tb, 0, kWave), tlast = __shfl(tb, 1, kWave);
-        const int64_t i = i0 + lane;
-        const bool valid = i <= ilast;
-        const int64_t t = valid ? edge_of(pre, i, tfirst, tlast) : tlast;
-        uint32_t m = 0;
-        if (valid) {
-            const int32_t s = pa[t], d = pb[t];
+        it.i = i0 + lane;
+        it.valid = it.i <= it.ilast;
+        it.t = it.valid ? edge_of(pre, it.i, tfirst, tlast) : tlast;
+        it.ll = 0;
+        it.x = 0;
+        if (it.valid) {
+            const int32_t s = pa[it.t], d = pb[it.t];
             const int32_t ls = len[s], ld = len[d];
-            const bool s_short = ls <= ld;
-            const int64_t sb = s_short ? boff[s] : boff[d];
-            const int64_t lb = s_short ? boff[d] : boff[s];
-            const int32_t ll = s_short ? ld : ls;
-            const int64_t k = i - pre[t];
-            const int32_t x = pool[sb + k];
-            const int64_t p = find_pos(pool, lb, ll, x);
-            if (p >= 0) {
-                const int fs = flag[sb + k], fl = flag[p];
-                const int bs = s_short ? fs : fl;
-                const int bd = s_short ? fl : fs;
-                const int cw = 2 - bs - bd;
-                if (cw) atomicAdd(&out[x], (uint32_t)(sign * cw));
-                local += sign * cw;
-                m = (bs | bd) ? 1u : 2u;
-            }
+            it.s_short = ls <= ld;
+            it.sb = it.s_short ? boff[s] : boff[d];
+            it.lb = it.s_short ? boff[d] : boff[s];
+            it.ll = it.s_short ? ld : ls;
+            it.k = it.i - pre[it.t];
+            it.x = pool[it.sb + it.k];
+        }
+    };
+    auto finish = [&](const Item &it, int32_t lo) {
+        uint32_t m = 0;
+        if (it.valid && lo < it.ll && pool[it.lb + lo] == it.x) {
+            const int fs = flag[it.sb + it.k], fl = flag[it.lb + lo];
+            const int bs = it.s_short ? fs : fl;
+            const int bd = it.s_short ? fl : fs;
+            const int cw = 2 - bs - bd;
+            if (cw) atomicAdd(&out[it.x], (uint32_t)(sign * cw));
+            local += sign * cw;
+            m = (bs | bd) ? 1u : 2u;
         }
         // segmented sum of m over lanes of the same edge (t is non-decreasing)
-        uint32_t x = m;
+        uint32_t xs = m;
 #pragma unroll
         for (int dd = 1; dd < kWave; dd <<= 1) {
-            const uint32_t y = __shfl_up(x, dd, kWave);
-            const int64_t ty = __shfl_up(t, dd, kWave);
-            if (lane >= dd && ty == t) x += y;
+            const uint32_t y = __shfl_up(xs, dd, kWave);
+            const int64_t ty = __shfl_up(it.t, dd, kWave);
+            if (lane >= dd && ty == it.t) xs += y;
         }
-        const int64_t tn = __shfl_down(t, 1, kWave);
-        const bool tail = lane == kWave - 1 || tn != t || i + 1 > ilast;
-        if (valid && tail && x) atomicAdd(&ecnt[t], x);
+        const int64_t tn = __shfl_down(it.t, 1, kWave);
+        const bool tail = lane == kWave - 1 || tn != it.t || it.i + 1 > it.ilast;
+        if (it.valid && tail && xs) atomicAdd(&ecnt[it.t], xs);
+    };
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += 2 * wstride) {
+        const int64_t c2 = c + wstride;
+        const bool two = c2 < nchunks;  // wave-uniform
+        Item a, b;
+        setup(c, a);
+        b.valid = false;
+        b.ll = 0;
+        if (two) setup(c2, b);
+        // interleaved lower_bound of a.x and b.x in their longer lists
+        int32_t lo0 = 0, hi0 = a.valid ? a.ll : 0, lo1 = 0, hi1 = b.valid ? b.ll : 0;
+        while (lo0 < hi0 || lo1 < hi1) {
+            const bool g0 = lo0 < hi0, g1 = lo1 < hi1;
+            const int32_t m0 = (lo0 + hi0) >> 1, m1 = (lo1 + hi1) >> 1;
+            const int32_t v0 = g0 ? pool[a.lb + m0] : 0;
+            const int32_t v1 = g1 ? pool[b.lb + m1] : 0;
+            if (g0) {
+                if (v0 < a.x) lo0 = m0 + 1; else hi0 = m0;
+            }
+            if (g1) {
+                if (v1 < b.x) lo1 = m1 + 1; else hi1 = m1;
+            }
+        }
+        finish(a, lo0);
+        if (two) finish(b, lo1);
     }
     const unsigned long long part[1] = {(unsigned long long)local};
     block_add_u64<1>(part, gsum);  // two's complement: a negative sum wraps
