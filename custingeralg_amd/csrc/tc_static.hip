@@ -936,11 +936,11 @@ struct CountArgs {
     // (bitmap tiles) or [3..5] (hash-table tiles)
     int32_t long_external;
     unsigned long long *long_stats;
-    // largest bin, oriented (every kernel of it): the learned table is not
-    // consulted and the fixed heuristic decides -- a task of a strategy other
-    // than hash makes k_count rebuild its job's tile, a per-job cost the
-    // per-task timings behind the table never see
-    int32_t no_learned_table;
+    // largest bin, oriented (every kernel of it): the fixed and learned
+    // policies stream every task through the tile -- a task of another
+    // strategy makes k_count rebuild its job's tile, a per-job cost the
+    // per-task timings behind the table never see (forced policies still hold)
+    int32_t hash_only;
 };
 
 __device__ inline int lg_bucket(int32_t x)
@@ -956,7 +956,8 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
     case TC_POLICY_BSEARCH: return S_BSEARCH;
     case TC_POLICY_HASH: return S_HASH;
     case TC_POLICY_LEARNED:
-        if (A.table && !A.no_learned_table) {
+        if (A.hash_only) return S_HASH;
+        if (A.table) {
             int i = min(lg_bucket(L), A.table_dim - 1);
             int j = min(lg_bucket(pl), A.table_dim - 1);
             return A.table[i * A.table_dim + j];
@@ -964,6 +965,7 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
         break;
     default: break;
     }
+    if (A.hash_only) return S_HASH;
     // fixed degree-ratio heuristic; merge only when ratio_x16 asks for it:
     // measured on 4096 batched R-MAT graphs, merging the short similar lists
     // it used to take (long <= 4 * short, long <= 96) cost 1.49x against
@@ -1671,12 +1673,15 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
         if (oriented) {
             // short tasks here, long hash tasks in k_count_long (bitmap and
             // hash-table tiles in separate launches for their LDS footprints)
-            // (the strategy rules must be the same in all three launches)
+            // (the strategy rules must be the same in all three launches; with
+            // every task hashed, k_count has nothing left to do)
             CountArgs Al = A;
-            Al.no_learned_table = 1;
+            Al.hash_only = 1;
             CountArgs As = Al;
             As.long_external = 1;
-            hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, As);
+            const bool forced = A.policy == TC_POLICY_MERGE || A.policy == TC_POLICY_BSEARCH;
+            if (forced)
+                hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, As);
             const int64_t gb = std::min<int64_t>(njobs, (int64_t)cus * 3);
             const int64_t gh = std::min<int64_t>(njobs, (int64_t)cus * 2);
             TC_HIP(hipEventRecord(ev_long[0], s));
