@@ -1449,8 +1449,14 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
 {
     constexpr int NBMAX = LMAX / 2;
     constexpr int kBmWords = 2 * NBMAX;  // uint2 words in HB's space
-    __shared__ int4 HB[NBMAX];
-    __shared__ uint32_t C[BM ? LMAX : 4 * NBMAX];
+    // one LDS block: the table (or bitmap) HB, then the counters C.  The
+    // hash-table launch reuses all of it for a wider bitmap when a tile's
+    // span allows: kWideWords words, then L counters
+    constexpr int kRawInt4 = BM ? NBMAX + LMAX / 4 : 2 * NBMAX;
+    constexpr int kWideWords = BM ? 0 : (kRawInt4 * 16 - LMAX * 4) / 8;
+    __shared__ int4 RAW[kRawInt4];
+    int4 *HB = RAW;
+    uint32_t *C = reinterpret_cast<uint32_t *>(RAW + NBMAX);
     __shared__ unsigned long long red[NW];
     __shared__ int32_t wsum[NW];
     const int lane = threadIdx.x & (kWave - 1);
@@ -1474,11 +1480,15 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;
         const BucketTable T{HB, nbb};
-        uint2 *BMw = reinterpret_cast<uint2 *>(HB);
+        uint2 *BMw = reinterpret_cast<uint2 *>(RAW);
         const TileBitmap BT{BMw, lo_v, span};
-        if (BM) {
+        // bitmap: always in the bitmap launch, and in the hash-table launch
+        // when the span fits the wide layout (counters after the words)
+        const bool use_bm = BM || nwords <= kWideWords;
+        uint32_t *Cb = BM ? C : reinterpret_cast<uint32_t *>(RAW) + 2 * kWideWords;
+        if (use_bm) {
             for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
-            for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
+            for (int i = threadIdx.x; i < L; i += nthr) Cb[i] = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < L; i += nthr) {
                 const uint32_t off = (uint32_t)(Sg[i] - lo_v);
@@ -1506,8 +1516,9 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
             __syncthreads();
         }
+        uint32_t *Ct = use_bm ? Cb : C;  // hit counters by slot
         auto probe = [&](const int32_t (&w)[kLongUnroll], int32_t (&slot)[kLongUnroll]) {
-            if (BM) {
+            if (use_bm) {
                 BT.find_n(w, slot);
             } else {
 #pragma unroll
@@ -1566,7 +1577,7 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
 #pragma unroll
                 for (int u = 0; u < kLongUnroll; ++u) {
                     const bool hit = slots[u] >= 0;
-                    if (hit) atomicAdd(&C[slots[u]], 1u);
+                    if (hit) atomicAdd(&Ct[slots[u]], 1u);
                     const uint32_t hc = (uint32_t)__popcll(__ballot(hit));
                     if (lane == cj[u]) mine_h += hc;
                 }
@@ -1592,9 +1603,9 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
         __syncthreads();
         unsigned long long hjob = 0;
         for (int w = 0; w < NW; ++w) hjob += red[w];
-        if (BM) {
+        if (use_bm) {
             for (int i = threadIdx.x; i < L; i += nthr) {
-                const uint32_t ci = C[i];
+                const uint32_t ci = Cb[i];
                 if (ci) atomicAdd(&A.out[A.order[Sg[i]]], 2u * ci);
             }
         } else {
