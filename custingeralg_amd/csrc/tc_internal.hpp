@@ -84,13 +84,21 @@ struct VInfo {
     int32_t len;
 };
 
-// One task of the oriented layout: task vertex a (rank) of an owner b, and the
+// One task of the oriented layout: for task vertex a of an owner b, the
 // suffix of N+(a) ranked above b -- the only entries that can close a
-// triangle -- as its start in the plus array and its length.
+// triangle -- as its start in the plus array (bits 24..63) and its length
+// (bits 0..23).  Under the degree order |N+(a)| <= sqrt(nnz), far below 2^24,
+// and the plus array has fewer than 2^40 entries.  a itself is the plus-array
+// row that holds entry base - 1.
 struct TaskRec {
-    int32_t a;
-    int32_t len;
-    int64_t base;
+    uint64_t v;
+
+    __host__ __device__ static TaskRec make(int64_t base, int32_t len)
+    {
+        return TaskRec{((uint64_t)base << 24) | (uint64_t)(uint32_t)len};
+    }
+    __host__ __device__ int64_t base() const { return (int64_t)(v >> 24); }
+    __host__ __device__ int32_t len() const { return (int32_t)(v & 0xFFFFFFu); }
 };
 
 // Intersection jobs: a slice [t_begin, t_end) of an owner's task list and the

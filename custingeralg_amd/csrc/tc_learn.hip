@@ -267,7 +267,8 @@ __global__ void k_model_table(const uint16_t *__restrict__ wb, int small_max, in
 // Samples entries of the oriented task array: owner (rank), task vertex
 // (rank) and the start of the task's useful suffix of N+(task).
 __global__ void k_sample_tasks(const int64_t *__restrict__ toff, const VInfo *__restrict__ vi,
-                               const TaskRec *__restrict__ tasks, int32_t nv, int64_t ntasks,
+                               const TaskRec *__restrict__ tasks, const int64_t *__restrict__ pcoff,
+                               int32_t nv, int64_t ntasks,
                                int64_t ncand, uint64_t seed, int32_t lmax,
                                int32_t *__restrict__ so, int32_t *__restrict__ st,
                                int32_t *__restrict__ sk)
@@ -289,9 +290,15 @@ __global__ void k_sample_tasks(const int64_t *__restrict__ toff, const VInfo *__
         const int32_t L = vb.len - vb.dm;
         if (L >= 1 && L <= lmax) {
             const TaskRec tk = tasks[e];
-            const int32_t a = tk.a;
-            const int32_t p = (vi[a].len - vi[a].dm) - tk.len - 1;  // position of the owner in N+(a)
-            if (tk.len >= 1) {
+            // the task vertex: the plus-array row holding entry base - 1
+            const int64_t x = tk.base() - 1;
+            int32_t a = 0, ah = nv - 1;
+            while (a < ah) {
+                const int32_t mid = a + ((ah - a + 1) >> 1);
+                if (pcoff[mid] <= x) a = mid; else ah = mid - 1;
+            }
+            const int32_t p = (vi[a].len - vi[a].dm) - tk.len() - 1;  // position of the owner in N+(a)
+            if (tk.len() >= 1) {
                 own = lo;
                 task = a;
                 kp = p;
@@ -485,7 +492,8 @@ int collect_timings(Graph &g, int64_t max_samples, uint64_t seed, float *feat, f
         (r = sk.ensure((size_t)ncand * 4)))
         return done(r);
     hipLaunchKernelGGL(k_sample_tasks, dim3(grid_for(ncand, 256, 4096)), dim3(256), 0, s,
-                       g.coff.as<int64_t>(), g.vinfo.as<VInfo>(), g.tasks.as<TaskRec>(), g.st.nv,
+                       g.coff.as<int64_t>(), g.vinfo.as<VInfo>(), g.tasks.as<TaskRec>(),
+                       g.ov[6].as<int64_t>(), g.st.nv,
                        g.ntasks, ncand, seed, kTimeLmax, so.as<int32_t>(), st.as<int32_t>(),
                        sk.as<int32_t>());
     std::vector<int32_t> ho(ncand), ht(ncand), hk(ncand);

@@ -578,7 +578,7 @@ __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int3
         }
         const int32_t slot = atomicAdd(&cursor[w], 1);
         if (slot < vi[w].dm)
-            tasks[toff[w] + slot] = TaskRec{r, (int32_t)(pcoff[r + 1] - e - 1), e + 1};
+            tasks[toff[w] + slot] = TaskRec::make(e + 1, (int32_t)(pcoff[r + 1] - e - 1));
         else
             atomicOr(bad, 1);  // rows are not symmetric
     }
@@ -645,7 +645,7 @@ __global__ void k_orient_transpose_hubs(
             if (padj[pa + mid] < w) lo = mid + 1; else hi = mid;
         }
         if (lo < pn && padj[pa + lo] == w)
-            tasks[toff[w] + slot] = TaskRec{a, (int32_t)(pn - lo - 1), pa + lo + 1};
+            tasks[toff[w] + slot] = TaskRec::make(pa + lo + 1, (int32_t)(pn - lo - 1));
         else
             atomicOr(bad, 1);  // rows are not symmetric
     }
@@ -751,7 +751,7 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
                 bool ok;
                 if (ORIENTED) {
                     // only the suffix of N+(a) ranked above the owner can hit
-                    pl = tasks[e].len;
+                    pl = tasks[e].len();
                     ok = pl > 0;
                 } else {
                     const int32_t a = adj[vb.rb + (e - coff[b])];
@@ -1059,8 +1059,8 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             bool other = false;
             for (int32_t t = J.t_begin + threadIdx.x; t < J.t_end; t += nthr) {
                 const TaskRec tk = A.tasks[tbase + t];
-                int32_t pl = tk.len;
-                int64_t pb = tk.base;
+                int32_t pl = tk.len();
+                int64_t pb = tk.base();
                 if (tiled && pl > 0) tile_filter(A.adj, pb, pl, lo_v, hi_v);
                 if (pl > 0 && choose_strategy(L, pl, A) != S_HASH) other = true;
             }
@@ -1130,9 +1130,8 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             if (valid) {
                 if (ORIENTED) {
                     const TaskRec tk = A.tasks[tbase + t];
-                    tv = tk.a;
-                    sl = tk.len;
-                    sb = tk.base;
+                    sl = tk.len();
+                    sb = tk.base();
                 } else {
                     tv = A.adj[tbase + t];
                 }
@@ -1534,8 +1533,8 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             int64_t pb = 0, sb = 0;
             if (t < J.t_end) {
                 const TaskRec tk = A.tasks[tbase + t];
-                pl = tk.len;
-                sb = tk.base;
+                pl = tk.len();
+                sb = tk.base();
                 pb = sb;
             }
             if (tiled && pl > 0) tile_filter(A.adj, pb, pl, lo_v, hi_v);
