@@ -1040,7 +1040,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
         const int32_t Lfull = ORIENTED ? vo.len - vo.dm : vo.len;
         const int32_t s0 = J.tile * LMAX;
         const int32_t L = min(LMAX, Lfull - s0);
-        const bool tiled = Lfull > LMAX;
+        const bool tiled = LMAX >= 4096 && Lfull > LMAX;  // smaller bins hold whole lists
         const int32_t *Sg = A.adj + obase + s0;  // owner tile (sorted) in global memory
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;         // buckets: 2^nbb >= L/2
@@ -1470,7 +1470,7 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
         const int32_t Lfull = vo.len - vo.dm;
         const int32_t s0 = J.tile * LMAX;
         const int32_t L = min(LMAX, Lfull - s0);
-        const bool tiled = Lfull > LMAX;
+        const bool tiled = LMAX >= 4096 && Lfull > LMAX;  // smaller bins hold whole lists
         const int32_t *Sg = A.adj + vo.rb + s0;
         const int32_t lo_v = Sg[0], hi_v = Sg[L - 1];
         const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
