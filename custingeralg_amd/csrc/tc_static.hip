@@ -1441,6 +1441,9 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
 // hash-table tiles.  Each skips the other's jobs; k_count<LMAX, NW, true>
 // with long_external does the tasks of the other strategies.
 constexpr int kLongUnroll = 8;
+// buffer resource word 3 for raw 32-bit loads on gfx9-family GPUs (stride 0,
+// range checked against num_records in bytes; out-of-range lanes read 0)
+constexpr int kBufFlags = 0x00020000;
 
 template <int LMAX, int NW, bool BM>
 __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job *__restrict__ jobs,
@@ -1475,18 +1478,18 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
         const int32_t lo_v = Sg[0], hi_v = Sg[L - 1];
         const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
         const int32_t nwords = (int32_t)((span + 31u) >> 5);
-        if ((nwords <= kBmWords) != BM) continue;  // the other instantiation's job
+        // (a tile needs one word past its span: the zero guard word)
+        if ((nwords < kBmWords) != BM) continue;  // the other instantiation's job
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;
         const BucketTable T{HB, nbb};
         uint2 *BMw = reinterpret_cast<uint2 *>(RAW);
-        const TileBitmap BT{BMw, lo_v, span};
         // bitmap: always in the bitmap launch, and in the hash-table launch
         // when the span fits the wide layout (counters after the words)
-        const bool use_bm = BM || nwords <= kWideWords;
+        const bool use_bm = BM || nwords < kWideWords;
         uint32_t *Cb = BM ? C : reinterpret_cast<uint32_t *>(RAW) + 2 * kWideWords;
         if (use_bm) {
-            for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
+            for (int i = threadIdx.x; i <= nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
             for (int i = threadIdx.x; i < L; i += nthr) Cb[i] = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < L; i += nthr) {
@@ -1504,7 +1507,7 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             int32_t run = incl - mine;
             for (int w = 0; w < wid; ++w) run += wsum[w];
             for (int32_t k = k0; k < k1; ++k) {
-                BMw[k].y = (uint32_t)run;
+                BMw[k].y = (uint32_t)run * 4u;
                 run += __popc(BMw[k].x);
             }
             __syncthreads();
@@ -1516,14 +1519,8 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             __syncthreads();
         }
         uint32_t *Ct = use_bm ? Cb : C;  // hit counters by slot
-        auto probe = [&](const int32_t (&w)[kLongUnroll], int32_t (&slot)[kLongUnroll]) {
-            if (use_bm) {
-                BT.find_n(w, slot);
-            } else {
-#pragma unroll
-                for (int u = 0; u < kLongUnroll; u += 2) T.find_n<2>(&w[u], &slot[u]);
-            }
-        };
+        // offsets past the span are clamped to the zero word at index nwords
+        const uint32_t guard = (uint32_t)nwords << 5;
         unsigned long long hits_job = 0;
 
         // the same round-robin deal of tasks to waves as k_count
@@ -1546,47 +1543,102 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             const int32_t ctot = __builtin_amdgcn_readlane(cincl, kWave - 1);
             if (ctot == 0) continue;
             const int32_t cexcl = cincl - myc;
-            // chunks c0 .. c0+kLongUnroll-1: their entries and owner lanes
-            auto load = [&](int32_t c0, int32_t (&w)[kLongUnroll], int32_t (&jj)[kLongUnroll]) {
+            // chunk c of lane j's task holds its suffix entries (c - cexcl_j) * 64 +
+            // lane.  Loads go through a buffer resource over the task's suffix:
+            // lanes past its end (and every chunk past ctot) read 0 from the range
+            // check, with no memory access, no branch and no exec mask, so the
+            // next batch's loads stay in flight while this one is probed.  0 is
+            // never in an owner's list (N+(b) ranks above b), so it never hits.
+            const int32_t lane4 = lane * 4;
+            // the loading side's task: the end of its chunk range (INT32_MAX
+            // once the last task is reached), the byte address of its next chunk
+            // and the bytes left from there.  Chunks are visited in order, so a
+            // task is entered at its first chunk.
+            int32_t lend = 0, lrem = 0;
+            uint64_t lp = 0;
+            auto load = [&](int32_t c0, int32_t (&w)[kLongUnroll]) {
 #pragma unroll
                 for (int u = 0; u < kLongUnroll; ++u) {
                     const int32_t c = c0 + u;
-                    w[u] = -1;
-                    jj[u] = -1;
-                    if (c < ctot) {
+                    if (c >= lend) {
                         const int j = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
-                        const int32_t cf = __builtin_amdgcn_readlane(cexcl, j);
-                        const int32_t n = __builtin_amdgcn_readlane(pl, j);
-                        const int32_t *lst = A.adj + readlane_i64(pb, j);
-                        const int32_t k = (c - cf) * kWave + lane;
-                        if (k < n) w[u] = lst[k];
-                        jj[u] = j;
+                        lend = __builtin_amdgcn_readlane(cincl, j);
+                        if (lend == ctot) lend = INT32_MAX;
+                        lrem = __builtin_amdgcn_readlane(pl, j) * 4;
+                        lp = reinterpret_cast<uint64_t>(A.adj + readlane_i64(pb, j));
                     }
+                    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                        reinterpret_cast<void *>(lp), 0, max(lrem, 0), kBufFlags);
+                    w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane4, 0, 0);
+                    lp += kWave * 4;
+                    lrem -= kWave * 4;
                 }
             };
-            uint32_t mine_h = 0;  // hits of this lane's own task
-            int32_t cur[kLongUnroll], cj[kLongUnroll];
-            load(0, cur, cj);
-            for (int32_t c0 = 0;; c0 += kLongUnroll) {
-                const bool more = c0 + kLongUnroll < ctot;
-                int32_t nxt[kLongUnroll], nj[kLongUnroll];
-                if (more) load(c0 + kLongUnroll, nxt, nj);
-                int32_t slots[kLongUnroll];
-                probe(cur, slots);
+            // the probing side's task (wave-uniform) and its hits so far; a
+            // task's chunks are consecutive, so its total is final when the
+            // chunk owner changes and goes to its lane of mine_h
+            uint32_t mine_h = 0;
+            int32_t pj = 0, pend = 0;
+            uint32_t pacc = 0;
+            // probes one batch (chunks c0 ..) and books its hits
+            auto probe = [&](int32_t c0, const int32_t (&w)[kLongUnroll]) {
+                uint32_t hc[kLongUnroll];
+                if (use_bm) {
+                    uint32_t off[kLongUnroll];
+                    uint2 x[kLongUnroll];
+#pragma unroll
+                    for (int u = 0; u < kLongUnroll; ++u) {
+                        off[u] = min((uint32_t)(w[u] - lo_v), guard);  // w < lo wraps high
+                        x[u] = BMw[off[u] >> 5];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLongUnroll; ++u) {
+                        // v_bfe_u32 reads bits 4:0 of its offset and width; the
+                        // word's prefix is kept in bytes (counter index * 4)
+                        const bool hit = __builtin_amdgcn_ubfe(x[u].x, off[u], 1u) != 0u;
+                        hc[u] = (uint32_t)__popcll(__ballot(hit));
+                        if (hit) {
+                            const uint32_t byte = x[u].y + 4u * __popc(__builtin_amdgcn_ubfe(x[u].x, 0u, off[u]));
+                            atomicAdd(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(Ct) + byte), 1u);
+                        }
+                    }
+                } else {
+                    int32_t slots[kLongUnroll];
+#pragma unroll
+                    for (int u = 0; u < kLongUnroll; u += 2) T.find_n<2>(&w[u], &slots[u]);
+#pragma unroll
+                    for (int u = 0; u < kLongUnroll; ++u) {
+                        const bool hit = slots[u] >= 0;
+                        hc[u] = (uint32_t)__popcll(__ballot(hit));
+                        if (hit) atomicAdd(&Ct[slots[u]], 1u);
+                    }
+                }
 #pragma unroll
                 for (int u = 0; u < kLongUnroll; ++u) {
-                    const bool hit = slots[u] >= 0;
-                    if (hit) atomicAdd(&Ct[slots[u]], 1u);
-                    const uint32_t hc = (uint32_t)__popcll(__ballot(hit));
-                    if (lane == cj[u]) mine_h += hc;
+                    const int32_t c = c0 + u;
+                    if (c >= pend) {  // INT32_MAX once the last task is reached
+                        if (pacc && lane == pj) mine_h = pacc;
+                        pj = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
+                        pend = __builtin_amdgcn_readlane(cincl, pj);
+                        if (pend == ctot) pend = INT32_MAX;
+                        pacc = 0;
+                    }
+                    pacc += hc[u];
                 }
-                if (!more) break;
-#pragma unroll
-                for (int u = 0; u < kLongUnroll; ++u) {
-                    cur[u] = nxt[u];
-                    cj[u] = nj[u];
-                }
+            };
+            // two batch buffers used in turn (no register copies, so the
+            // batch being loaded is never waited for before it is probed)
+            int32_t ba[kLongUnroll], bb[kLongUnroll];
+            load(0, ba);
+            for (int32_t c0 = 0;; c0 += 2 * kLongUnroll) {
+                load(c0 + kLongUnroll, bb);  // all zeros past ctot
+                probe(c0, ba);
+                if (c0 + kLongUnroll >= ctot) break;
+                load(c0 + 2 * kLongUnroll, ba);
+                probe(c0 + kLongUnroll, bb);
+                if (c0 + 2 * kLongUnroll >= ctot) break;
             }
+            if (pacc && lane == pj) mine_h = pacc;
             if (mine_h) {
                 hits_job += mine_h;
                 // a tiled owner meets the task once per tile
