@@ -31,7 +31,21 @@ namespace tc {
 
 enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2 };
 constexpr int kUnroll = 4;
-constexpr int kBigTask = 64;  // hash tasks at least this long get the whole wave
+#ifndef TC_BIG_TASK
+#define TC_BIG_TASK 32
+#endif
+#ifndef TC_BIG_TASK_SMALL
+#define TC_BIG_TASK_SMALL (1 << 30)
+#endif
+// hash tasks at least this long are streamed in wave-wide chunks (the small
+// bin streams none: its lists are short and the streaming code's registers
+// cost its other tasks more than streaming saves; the largest bin, where
+// k_count takes hash tasks only under forced policies, walks tasks of 64 or
+// more one at a time)
+template <int LMAX> constexpr int big_task_min()
+{
+    return LMAX <= 64 ? TC_BIG_TASK_SMALL : LMAX >= 4096 ? 64 : TC_BIG_TASK;
+}
 #ifndef TC_BIG_UNROLL
 #define TC_BIG_UNROLL 8
 #endif
@@ -977,6 +991,94 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
     return S_HASH;
 }
 
+// buffer resource word 3 for raw 32-bit loads on gfx9-family GPUs (stride 0,
+// range checked against num_records in bytes; out-of-range lanes read 0)
+constexpr int kBufFlags = 0x00020000;
+
+// Streams the task lists of one wave's 64 tasks (lane t's list: pl entries at
+// adj[pb ..]) as one sequence of 64-entry chunks: chunk c belongs to the first
+// lane whose inclusive chunk count exceeds c, so chunk owners are wave-uniform
+// and a batch of U chunks spans several short tasks.  Loads go through a buffer
+// resource over the owning task's remaining entries: lanes past its end (and
+// every chunk past the last) read 0 from the range check, with no memory
+// access, no branch and no exec mask.  Two batches of registers are used in
+// turn, so the next batch's loads are in flight while a batch is probed.  The
+// owning task is tracked in scalar registers and looked up again (one ballot,
+// a few readlanes) only when a chunk passes its end.  probe(w, hc) probes a
+// batch and sets hc[u] to the hits of chunk u over the wave.  ZERO_SAFE: 0 is
+// never in the probed set (the oriented layout, whose lists hold ranks above
+// their owner); otherwise lanes past a task's end read -1.  Returns the hits of
+// this lane's task.  Called by the whole wave.
+template <int U, bool ZERO_SAFE, class Probe>
+__device__ inline uint32_t stream_tasks(const int32_t *adj, int32_t pl, int64_t pb, Probe &&probe)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int32_t myc = (pl + kWave - 1) / kWave;
+    const int32_t cincl = wave_incl_scan_i32(myc);
+    const int32_t ctot = __builtin_amdgcn_readlane(cincl, kWave - 1);
+    if (ctot == 0) return 0;
+    const int32_t lane4 = lane * 4;
+    // the loading side's task: the end of its chunk range (INT32_MAX once the
+    // last task is reached), the byte address of its next chunk and the bytes
+    // left from there.  Chunks are visited in order, so a task is entered at
+    // its first chunk.
+    int32_t lend = 0, lrem = 0;
+    uint64_t lp = 0;
+    auto load = [&](int32_t c0, int32_t (&w)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t c = c0 + u;
+            if (c >= lend) {
+                const int j = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
+                lend = __builtin_amdgcn_readlane(cincl, j);
+                if (lend == ctot) lend = INT32_MAX;
+                lrem = __builtin_amdgcn_readlane(pl, j) * 4;
+                lp = reinterpret_cast<uint64_t>(adj + readlane_i64(pb, j));
+            }
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(lp), 0, max(lrem, 0), kBufFlags);
+            int32_t v = __builtin_amdgcn_raw_buffer_load_b32(rs, lane4, 0, 0);
+            if (!ZERO_SAFE && lane4 >= lrem) v = -1;
+            w[u] = v;
+            lp += kWave * 4;
+            lrem -= kWave * 4;
+        }
+    };
+    // the probing side's task and its hits so far; a task's chunks are
+    // consecutive, so its total is final when the chunk owner changes
+    uint32_t mine_h = 0, pacc = 0;
+    int32_t pj = 0, pend = 0;
+    auto book = [&](int32_t c0, const uint32_t (&hc)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t c = c0 + u;
+            if (c >= pend) {  // INT32_MAX once the last task is reached
+                if (pacc && lane == pj) mine_h = pacc;
+                pj = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
+                pend = __builtin_amdgcn_readlane(cincl, pj);
+                if (pend == ctot) pend = INT32_MAX;
+                pacc = 0;
+            }
+            pacc += hc[u];
+        }
+    };
+    int32_t ba[U], bb[U];
+    uint32_t hc[U];
+    load(0, ba);
+    for (int32_t c0 = 0;; c0 += 2 * U) {
+        load(c0 + U, bb);  // all zeros past the last chunk
+        probe(ba, hc);
+        book(c0, hc);
+        if (c0 + U >= ctot) break;
+        load(c0 + 2 * U, ba);
+        probe(bb, hc);
+        book(c0 + U, hc);
+        if (c0 + 2 * U >= ctot) break;
+    }
+    if (pacc && lane == pj) mine_h = pacc;
+    return mine_h;
+}
+
 // The part of a task list [pb, pb+pl) inside an owner tile's id range
 // [lo_v, hi_v] (tiled owners): two binary searches.
 __device__ inline void tile_filter(const int32_t *__restrict__ adj, int64_t &pb, int32_t &pl,
@@ -1175,11 +1277,9 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             wave_sync();
 
             // Long hash tasks are streamed in 64-entry chunks across the whole
-            // wave: chunk c belongs to the task (lane) whose inclusive chunk
-            // count first exceeds c, so one batch of kBigUnroll loads per lane
-            // spans several tasks.  Task bases and lengths are wave-uniform
-            // (readlane), hit counts come from ballots.
-            const bool is_big = strat == S_HASH && pl >= kBigTask;
+            // wave (stream_tasks): one batch of kBigUnroll loads per lane spans
+            // several tasks, and hit counts come from ballots.
+            const bool is_big = strat == S_HASH && pl >= big_task_min<LMAX>();
             if (!kChunkedLong) {
                 // largest bin: one long task at a time across the wave
                 uint64_t big = __ballot(is_big);
@@ -1212,44 +1312,18 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
                     if (lane == jb2) tcnt[wid][jb2] = h;
                 }
             } else if (__ballot(is_big)) {
-                const int32_t myc = is_big ? (pl + kWave - 1) / kWave : 0;
-                const int32_t cincl = wave_incl_scan_i32(myc);
-                const int32_t ctot = __builtin_amdgcn_readlane(cincl, kWave - 1);
-                int32_t cur_j = -1;
-                uint32_t cur_h = 0;
-                for (int32_t c0 = 0; c0 < ctot; c0 += kBigUnroll) {
-                    int32_t w[kBigUnroll], jj[kBigUnroll];
+                const uint32_t h = stream_tasks<kBigUnroll, ORIENTED>(
+                    A.adj, is_big ? pl : 0, pb, [&](const int32_t (&w)[kBigUnroll], uint32_t (&hc)[kBigUnroll]) {
+                        int32_t slots[kBigUnroll];
+                        lookup_n(w, slots);
 #pragma unroll
-                    for (int u = 0; u < kBigUnroll; ++u) {
-                        const int32_t c = c0 + u;
-                        w[u] = -1;
-                        jj[u] = -1;
-                        if (c < ctot) {
-                            const int j = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
-                            const int32_t cfirst = __builtin_amdgcn_readlane(cincl - myc, j);
-                            const int32_t n = __builtin_amdgcn_readlane(pl, j);
-                            const int32_t *lst = A.adj + readlane_i64(pb, j);
-                            const int32_t k = (c - cfirst) * kWave + lane;
-                            if (k < n) w[u] = lst[k];
-                            jj[u] = j;
+                        for (int u = 0; u < kBigUnroll; ++u) {
+                            const bool hit = slots[u] >= 0;
+                            hc[u] = (uint32_t)__popcll(__ballot(hit));
+                            if (ORIENTED && hit) atomicAdd(&C[slots[u]], 1u);
                         }
-                    }
-                    int32_t slots[kBigUnroll];
-                    lookup_n(w, slots);
-#pragma unroll
-                    for (int u = 0; u < kBigUnroll; ++u) {
-                        const int32_t slot = slots[u];
-                        if (ORIENTED && slot >= 0) atomicAdd(&C[slot], 1u);
-                        const uint32_t hc = (uint32_t)__popcll(__ballot(slot >= 0));
-                        if (jj[u] != cur_j) {
-                            if (cur_j >= 0 && lane == 0) tcnt[wid][cur_j] = cur_h;
-                            cur_j = jj[u];
-                            cur_h = 0;
-                        }
-                        cur_h += hc;
-                    }
-                }
-                if (cur_j >= 0 && lane == 0) tcnt[wid][cur_j] = cur_h;
+                    });
+                if (is_big) mcount = h;
             }
 
             // Flattened phases over the remaining tasks of this wave: items of
@@ -1260,7 +1334,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             for (int phase = 0; phase < 2; ++phase) {
                 const int want = phase == 0 ? S_HASH : S_BSEARCH;
                 const int32_t items =
-                    (strat == want && !(phase == 0 && pl >= kBigTask)) ? (phase == 0 ? pl : L) : 0;
+                    (strat == want && !(phase == 0 && pl >= big_task_min<LMAX>())) ? (phase == 0 ? pl : L) : 0;
                 const int32_t incl = wave_incl_scan_i32(items);
                 wave_sync();
                 pre[wid][lane + 1] = incl;
@@ -1441,9 +1515,6 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
 // hash-table tiles.  Each skips the other's jobs; k_count<LMAX, NW, true>
 // with long_external does the tasks of the other strategies.
 constexpr int kLongUnroll = 8;
-// buffer resource word 3 for raw 32-bit loads on gfx9-family GPUs (stride 0,
-// range checked against num_records in bytes; out-of-range lanes read 0)
-constexpr int kBufFlags = 0x00020000;
 
 template <int LMAX, int NW, bool BM>
 __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job *__restrict__ jobs,
@@ -1538,51 +1609,9 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             if (!(pl > 0 && choose_strategy(L, pl, A) == S_HASH)) pl = 0;
             n_hash += pl > 0;
             n_items += (unsigned long long)pl;
-            const int32_t myc = (pl + kWave - 1) / kWave;
-            const int32_t cincl = wave_incl_scan_i32(myc);
-            const int32_t ctot = __builtin_amdgcn_readlane(cincl, kWave - 1);
-            if (ctot == 0) continue;
-            const int32_t cexcl = cincl - myc;
-            // chunk c of lane j's task holds its suffix entries (c - cexcl_j) * 64 +
-            // lane.  Loads go through a buffer resource over the task's suffix:
-            // lanes past its end (and every chunk past ctot) read 0 from the range
-            // check, with no memory access, no branch and no exec mask, so the
-            // next batch's loads stay in flight while this one is probed.  0 is
-            // never in an owner's list (N+(b) ranks above b), so it never hits.
-            const int32_t lane4 = lane * 4;
-            // the loading side's task: the end of its chunk range (INT32_MAX
-            // once the last task is reached), the byte address of its next chunk
-            // and the bytes left from there.  Chunks are visited in order, so a
-            // task is entered at its first chunk.
-            int32_t lend = 0, lrem = 0;
-            uint64_t lp = 0;
-            auto load = [&](int32_t c0, int32_t (&w)[kLongUnroll]) {
-#pragma unroll
-                for (int u = 0; u < kLongUnroll; ++u) {
-                    const int32_t c = c0 + u;
-                    if (c >= lend) {
-                        const int j = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
-                        lend = __builtin_amdgcn_readlane(cincl, j);
-                        if (lend == ctot) lend = INT32_MAX;
-                        lrem = __builtin_amdgcn_readlane(pl, j) * 4;
-                        lp = reinterpret_cast<uint64_t>(A.adj + readlane_i64(pb, j));
-                    }
-                    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                        reinterpret_cast<void *>(lp), 0, max(lrem, 0), kBufFlags);
-                    w[u] = __builtin_amdgcn_raw_buffer_load_b32(rs, lane4, 0, 0);
-                    lp += kWave * 4;
-                    lrem -= kWave * 4;
-                }
-            };
-            // the probing side's task (wave-uniform) and its hits so far; a
-            // task's chunks are consecutive, so its total is final when the
-            // chunk owner changes and goes to its lane of mine_h
-            uint32_t mine_h = 0;
-            int32_t pj = 0, pend = 0;
-            uint32_t pacc = 0;
-            // probes one batch (chunks c0 ..) and books its hits
-            auto probe = [&](int32_t c0, const int32_t (&w)[kLongUnroll]) {
-                uint32_t hc[kLongUnroll];
+            // 0 never hits (owner lists hold ranks above their owner): it
+            // clamps to the zero guard word, and the hash table holds no 0
+            const uint32_t mine_h = stream_tasks<kLongUnroll, true>(A.adj, pl, pb, [&](const int32_t (&w)[kLongUnroll], uint32_t (&hc)[kLongUnroll]) {
                 if (use_bm) {
                     uint32_t off[kLongUnroll];
                     uint2 x[kLongUnroll];
@@ -1613,32 +1642,7 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
                         if (hit) atomicAdd(&Ct[slots[u]], 1u);
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < kLongUnroll; ++u) {
-                    const int32_t c = c0 + u;
-                    if (c >= pend) {  // INT32_MAX once the last task is reached
-                        if (pacc && lane == pj) mine_h = pacc;
-                        pj = __ffsll((unsigned long long)__ballot(cincl > c)) - 1;
-                        pend = __builtin_amdgcn_readlane(cincl, pj);
-                        if (pend == ctot) pend = INT32_MAX;
-                        pacc = 0;
-                    }
-                    pacc += hc[u];
-                }
-            };
-            // two batch buffers used in turn (no register copies, so the
-            // batch being loaded is never waited for before it is probed)
-            int32_t ba[kLongUnroll], bb[kLongUnroll];
-            load(0, ba);
-            for (int32_t c0 = 0;; c0 += 2 * kLongUnroll) {
-                load(c0 + kLongUnroll, bb);  // all zeros past ctot
-                probe(c0, ba);
-                if (c0 + kLongUnroll >= ctot) break;
-                load(c0 + 2 * kLongUnroll, ba);
-                probe(c0 + kLongUnroll, bb);
-                if (c0 + 2 * kLongUnroll >= ctot) break;
-            }
-            if (pacc && lane == pj) mine_h = pacc;
+            });
             if (mine_h) {
                 hits_job += mine_h;
                 // a tiled owner meets the task once per tile
