@@ -47,7 +47,7 @@ template <int LMAX> constexpr int big_task_min()
     return LMAX <= 64 ? TC_BIG_TASK_SMALL : LMAX >= 4096 ? 64 : TC_BIG_TASK;
 }
 #ifndef TC_BIG_UNROLL
-#define TC_BIG_UNROLL 8
+#define TC_BIG_UNROLL 4
 #endif
 constexpr int kBigUnroll = TC_BIG_UNROLL;  // loads in flight per lane on long tasks
 
@@ -569,7 +569,7 @@ __global__ void k_narrow_u64(const uint64_t *__restrict__ y, int64_t n, int32_t 
 // length of the suffix of N+(r) after w, the only part that can hit.  The
 // order of an owner's tasks is arbitrary.
 __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int32_t *__restrict__ pcrow,
-                                   const int32_t *__restrict__ padj, const VInfo *__restrict__ vi,
+                                   const int32_t *__restrict__ padj,
                                    const int64_t *__restrict__ toff, int32_t nv, int64_t nplus,
                                    TaskRec *__restrict__ tasks, int32_t *__restrict__ cursor,
                                    unsigned long long *__restrict__ sc, int32_t *__restrict__ bad)
@@ -591,8 +591,11 @@ __global__ void k_orient_transpose(const int64_t *__restrict__ pcoff, const int3
             continue;
         }
         const int32_t slot = atomicAdd(&cursor[w], 1);
-        if (slot < vi[w].dm)
-            tasks[toff[w] + slot] = TaskRec::make(e + 1, (int32_t)(pcoff[r + 1] - e - 1));
+        // |N-(w)| from the task offsets (one cache line with toff[w]) rather
+        // than from vi[w], a third random read per entry
+        const int64_t t0 = toff[w];
+        if (slot < toff[w + 1] - t0)
+            tasks[t0 + slot] = TaskRec::make(e + 1, (int32_t)(pcoff[r + 1] - e - 1));
         else
             atomicOr(bad, 1);  // rows are not symmetric
     }
@@ -1948,7 +1951,7 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
                        (int64_t)nv, 0u);
     if (nplus > 0) {
         hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 8192)), dim3(256), 0, s,
-                           pcoff, pcrow, padj, vi, toff, nv, nplus, g.tasks.as<TaskRec>(), cursor,
+                           pcoff, pcrow, padj, toff, nv, nplus, g.tasks.as<TaskRec>(), cursor,
                            sc, bad);
         hipLaunchKernelGGL(k_orient_transpose_hubs, dim3(cus * 16), dim3(256), 0, s, pcoff, toff,
                            order, rnk, ocoff, g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), mask,
