@@ -1356,7 +1356,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
                         ok[u] = it < total;
                         jj[u] = j;
                         kk[u] = 0;
-                        ww[u] = -1;
+                        const int32_t *src = A.adj;  // a valid address for lanes past the end
                         if (ok[u]) {
                             if (pre[wid][j + 1] <= it) {
                                 int32_t lo = j + 1, hi = kWave;  // largest j with pre[j] <= it
@@ -1368,8 +1368,12 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
                             }
                             jj[u] = j;
                             kk[u] = it - pre[wid][j];
-                            ww[u] = (phase == 0) ? A.adj[tpb[wid][j] + kk[u]] : Sg[kk[u]];
+                            src = (phase == 0) ? A.adj + tpb[wid][j] + kk[u] : Sg + kk[u];
                         }
+                        // the load sits outside the branch: loads under an exec
+                        // mask make the compiler wait for all of them at once
+                        const int32_t x = *src;
+                        ww[u] = ok[u] ? x : -1;
                     }
                     if (phase == 0) {
                         int32_t slots[kUnroll];
