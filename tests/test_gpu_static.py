@@ -89,13 +89,19 @@ def test_er16_plain_config():
 
 
 @pytest.mark.parametrize("scale", [12, 16])
-def test_rmat_against_reference_formula(scale):
+@pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
+def test_rmat_against_reference_formula(scale, method):
+    # DIRECT streams the mid bin's long neighbour lists in wave-wide chunks
+    # whose lanes past a list's end must read -1, not 0: vertex 0 is the
+    # R-MAT hub and sits in most lists
     off, adj = O.rmat_graph(scale, 16, 2)
     ref = O.custinger_count(off, adj)
     for policy in POLICIES:
-        pv, glob, _ = count(off, adj, policy=policy)
+        pv, glob, st = count(off, adj, policy=policy, method=method)
         assert np.array_equal(pv, ref)
         assert glob == int(ref.astype(np.uint64).sum())
+        if method == T.METHOD_DIRECT:
+            assert st["method_used"] == T.METHOD_DIRECT
 
 
 def test_rmat18_schedules_identical():
@@ -110,6 +116,20 @@ def test_rmat18_schedules_identical():
         pv, glob, _ = count(off, adj, **v)
         assert np.array_equal(pv, want), v
         assert glob == 6 * Tt
+
+
+@pytest.mark.parametrize("n", [130, 300, 600])
+@pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
+def test_complete_graphs_chunk_boundaries(n, method):
+    """K_n: the oriented task suffixes take every length up to n - 2, so the
+    wave-wide chunk streams end on and off 64-entry boundaries in every bin;
+    the direct rows all hold vertex 0."""
+    off, adj, t = GR.complete_fast(n)
+    pv, glob, st = count(off, adj, method=method)
+    assert np.all(pv == (2 * t).astype(np.uint32))
+    assert glob == 2 * int(t.sum())
+    if method == T.METHOD_DIRECT:
+        assert st["method_used"] == T.METHOD_DIRECT
 
 
 def test_tiled_owner_lists():
