@@ -30,7 +30,7 @@
 namespace tc {
 
 enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2 };
-constexpr int kUnroll = 4;
+constexpr int kUnroll = 2;
 #ifndef TC_BIG_TASK
 #define TC_BIG_TASK 32
 #endif
