@@ -160,12 +160,14 @@ struct BucketTable {
 // word i holds the bits of values lo+32i .. lo+32i+31 (x) and the number of
 // set bits in words < i (y), so the position of w in the tile (its counter
 // slot) is y plus the set bits below w in x.  One 8-byte LDS read per probe.
-// find_n probes a batch branch-free: all reads are issued before any is used
-// (out-of-range values read word 0 and are discarded).
+// Offsets are clamped to guard = 32 * (number of words): values outside the
+// range (w < lo and w = -1 wrap high) read the zero word after the last one,
+// so a probe needs no range test.  find_n probes a batch branch-free: all
+// reads are issued before any is used.
 struct TileBitmap {
     const uint2 *BM;
     int32_t lo;
-    uint32_t span;
+    uint32_t guard;
 
     template <int U>
     __device__ inline void find_n(const int32_t (&w)[U], int32_t (&slot)[U]) const
@@ -174,15 +176,14 @@ struct TileBitmap {
         uint32_t off[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            off[u] = (uint32_t)(w[u] - lo);  // w < lo (and w = -1) wraps past span
-            x[u] = BM[off[u] < span ? (off[u] >> 5) : 0u];
+            off[u] = min((uint32_t)(w[u] - lo), guard);
+            x[u] = BM[off[u] >> 5];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            // bitwise tests, no short-circuit: keeps the batch free of branches
-            const uint32_t b = off[u] & 31u;
-            const uint32_t hit = (uint32_t)(off[u] < span) & ((x[u].x >> b) & 1u);
-            const int32_t pos = (int32_t)(x[u].y + __popc(x[u].x & ((1u << b) - 1u)));
+            // v_bfe_u32 reads bits 4:0 of its offset and width
+            const bool hit = __builtin_amdgcn_ubfe(x[u].x, off[u], 1u) != 0u;
+            const int32_t pos = (int32_t)(x[u].y + __popc(__builtin_amdgcn_ubfe(x[u].x, 0u, off[u])));
             slot[u] = hit ? pos : -1;
         }
     }
@@ -1155,9 +1156,10 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
         // the hash table (same LDS space)
         const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
         const int32_t nwords = (int32_t)((span + 31u) >> 5);
-        const bool use_bm = kUseBitmap && nwords <= kBmWords;
+        // (one word past the span: the zero guard word)
+        const bool use_bm = kUseBitmap && nwords < kBmWords;
         uint2 *BMw = reinterpret_cast<uint2 *>(HB);
-        const TileBitmap BT{BMw, lo_v, span};
+        const TileBitmap BT{BMw, lo_v, (uint32_t)nwords << 5};
         if (ORIENTED && !kChunkedLong && A.long_external) {
             // k_count_long takes every hash task: only a job holding a task of
             // another strategy needs its tile here
@@ -1172,7 +1174,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             if (!__syncthreads_or(other)) continue;
         }
         if (use_bm) {
-            for (int i = threadIdx.x; i < nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
+            for (int i = threadIdx.x; i <= nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
             if (ORIENTED)
                 for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
             __syncthreads();
