@@ -959,6 +959,7 @@ struct CountArgs {
     // strategy makes k_count rebuild its job's tile, a per-job cost the
     // per-task timings behind the table never see (forced policies still hold)
     int32_t hash_only;
+    unsigned long long *job_ctr;  // zeroed job counter of the largest bin's bitmap launch
 };
 
 __device__ inline int lg_bucket(int32_t x)
@@ -1138,6 +1139,9 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
     unsigned long long hits_all = 0;
     unsigned long long sc[3] = {0, 0, 0};
 
+    // a fixed round-robin deal of jobs: a device counter per job (as in
+    // k_count_long's bitmap launch) serialises on one address at these jobs'
+    // sizes (small bin 3.3 -> 23 ms at R-MAT 24)
     for (int64_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
         const Job J = jobs[jb];
         const VInfo vo = A.vi[J.owner];
@@ -1546,7 +1550,23 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
     const int nthr = NW * kWave;
     unsigned long long hits_all = 0, n_hash = 0, n_items = 0, n_tile = 0;
 
-    for (int64_t jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+    // The bitmap launch takes its jobs from a device counter, so blocks that
+    // drew cheap jobs take more and the launch ends together (a fixed
+    // round-robin deal left a long tail: 105 -> 88 ms at R-MAT 24).  The
+    // hash-table launch, which skips three jobs in four, keeps the fixed deal.
+    __shared__ int64_t s_job;
+    int64_t jnext = blockIdx.x;
+    for (;;) {
+        int64_t jb = jnext;
+        if (BM) {
+            if (threadIdx.x == 0) s_job = (int64_t)atomicAdd(A.job_ctr, 1ull);
+            __syncthreads();
+            jb = s_job;
+            __syncthreads();
+        } else {
+            jnext += gridDim.x;
+        }
+        if (jb >= njobs) break;
         const Job J = jobs[jb];
         const VInfo vo = A.vi[J.owner];
         const int64_t tbase = A.toff[J.owner];
@@ -2006,7 +2026,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     bool oriented = method == TC_METHOD_ORIENTED;
 
     // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin tasks
-    // and bytes, [10..12] items, [13..15] tile items, [16..21] long-task stats
+    // and bytes, [10..12] items, [13..15] tile items, [16..21] long-task stats,
+    // [22] job counter of the largest bin's bitmap launch
     TC_TRY(g.accum.ensure(24 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
     uint32_t *outp = out;
@@ -2119,6 +2140,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.global = acc;
     A.strat = acc + 1;
     A.long_stats = acc + 16;
+    A.job_ctr = acc + 22;
     A.policy = sched.policy;
     A.ratio_x16 = std::max(sched.ratio_x16, 0);
     A.skip_lower = skip_lower ? 1 : 0;
