@@ -135,6 +135,7 @@ struct Graph {
     // pairs (a, position of b in the rank-sorted N+(a)) for a in N-(b);
     // ov = scratch of the orientation
     DevBuf order, rnk, tasks;
+    DevBuf hjobs;  // the largest bin's jobs left to its hash-table launch
     DevBuf ov[13];
     int64_t ntasks = 0;  // entries of the task array (oriented layout)
     int64_t nplus = 0;   // entries of the plus array (oriented layout)

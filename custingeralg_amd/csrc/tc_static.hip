@@ -959,7 +959,11 @@ struct CountArgs {
     // strategy makes k_count rebuild its job's tile, a per-job cost the
     // per-task timings behind the table never see (forced policies still hold)
     int32_t hash_only;
-    unsigned long long *job_ctr;  // zeroed job counter of the largest bin's bitmap launch
+    // zeroed job counters of the largest bin's launches: [0] the bitmap
+    // launch's next job, [1] the length of the list hjobs of the jobs it left
+    // to the hash-table launch, [2] that launch's next entry of the list
+    unsigned long long *job_ctr;
+    int64_t *hjobs;  // room for every job of the largest bin (or null)
 };
 
 __device__ inline int lg_bucket(int32_t x)
@@ -1550,16 +1554,23 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
     const int nthr = NW * kWave;
     unsigned long long hits_all = 0, n_hash = 0, n_items = 0, n_tile = 0;
 
-    // The bitmap launch takes its jobs from a device counter, so blocks that
-    // drew cheap jobs take more and the launch ends together (a fixed
-    // round-robin deal left a long tail: 105 -> 88 ms at R-MAT 24).  The
-    // hash-table launch, which skips three jobs in four, keeps the fixed deal.
+    // Jobs come from device counters, so blocks that drew cheap jobs take
+    // more and a launch ends together (a fixed round-robin deal left a long
+    // tail: 105 -> 88 ms at R-MAT 24).  The bitmap launch walks every job and
+    // lists the ones it leaves to the hash-table launch, which walks that list.
     __shared__ int64_t s_job;
     int64_t jnext = blockIdx.x;
     for (;;) {
         int64_t jb = jnext;
-        if (BM) {
-            if (threadIdx.x == 0) s_job = (int64_t)atomicAdd(A.job_ctr, 1ull);
+        if (BM || A.hjobs) {
+            if (threadIdx.x == 0) {
+                if (BM) {
+                    s_job = (int64_t)atomicAdd(A.job_ctr, 1ull);
+                } else {
+                    const int64_t k = (int64_t)atomicAdd(A.job_ctr + 2, 1ull);
+                    s_job = k < (int64_t)A.job_ctr[1] ? A.hjobs[k] : njobs;
+                }
+            }
             __syncthreads();
             jb = s_job;
             __syncthreads();
@@ -1579,7 +1590,10 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
         const uint32_t span = (uint32_t)(hi_v - lo_v) + 1u;
         const int32_t nwords = (int32_t)((span + 31u) >> 5);
         // (a tile needs one word past its span: the zero guard word)
-        if ((nwords < kBmWords) != BM) continue;  // the other instantiation's job
+        if ((nwords < kBmWords) != BM) {  // the other instantiation's job
+            if (BM && A.hjobs && threadIdx.x == 0) A.hjobs[atomicAdd(A.job_ctr + 1, 1ull)] = jb;
+            continue;
+        }
         int nbb = 0;
         while ((1 << nbb) * 2 < L) ++nbb;
         const BucketTable T{HB, nbb};
@@ -2027,8 +2041,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
 
     // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin tasks
     // and bytes, [10..12] items, [13..15] tile items, [16..21] long-task stats,
-    // [22] job counter of the largest bin's bitmap launch
-    TC_TRY(g.accum.ensure(24 * sizeof(unsigned long long)));
+    // [22..24] job counters of the largest bin's two launches
+    TC_TRY(g.accum.ensure(32 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
     uint32_t *outp = out;
     if (!outp) {
@@ -2037,7 +2051,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     }
 
     TC_HIP(hipEventRecord(g.ev[0], s));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, acc, (int64_t)24);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, acc, (int64_t)32);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s, outp,
                        (int64_t)nv, 0u);
     TC_HIP(hipGetLastError());
@@ -2168,6 +2182,10 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nplus, 256, 65536)), dim3(256), 0, s,
                            g.pcnt.as<uint32_t>(), nplus, 0u);
         A.pcnt = g.pcnt.as<uint32_t>();
+    }
+    if (oriented && jt[2] > 0) {
+        TC_TRY(g.hjobs.ensure((size_t)jt[2] * sizeof(int64_t)));
+        A.hjobs = g.hjobs.as<int64_t>();
     }
     const Job *jl[3] = {j0, j1, j2};
     for (int b = 0; b < 3; ++b) {
