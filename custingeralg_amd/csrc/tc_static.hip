@@ -1522,15 +1522,14 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
 }
 
 // Hash-strategy tasks of the largest bin, oriented method (almost all of the
-// bin's items).  A wave's 64 tasks are streamed as one sequence of 64-entry
-// chunks (chunk c belongs to the first task whose inclusive chunk count
-// exceeds c; chunk owners are wave-uniform), kLongUnroll chunks per batch, and
-// the next batch's loads are issued before the current batch is probed, so
-// two batches are in flight per wave while the LDS probes run.  BM = true
-// takes the jobs whose owner tile fits the bitmap and needs only the bitmap
-// and L counters in LDS (three blocks per CU); BM = false takes the
-// hash-table tiles.  Each skips the other's jobs; k_count<LMAX, NW, true>
-// with long_external does the tasks of the other strategies.
+// bin's items).  A wave's 64 tasks are streamed by stream_tasks as one
+// sequence of 64-entry chunks, kLongUnroll chunks per batch, with the next
+// batch's loads in flight while the current one is probed.  BM = true walks
+// every job, takes those whose owner tile fits the bitmap (it needs only the
+// bitmap and L counters in LDS: three blocks per CU) and lists the others for
+// BM = false, which takes the hash-table and wide-bitmap tiles from that list;
+// k_count<LMAX, NW, true> with long_external does the tasks of the other
+// strategies.
 constexpr int kLongUnroll = 8;
 
 template <int LMAX, int NW, bool BM>
