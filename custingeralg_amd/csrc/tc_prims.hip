@@ -183,64 +183,48 @@ __global__ void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int s
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: keys of a tile are ranked in tile order, round by round,
-// into the tile's digit-sorted order in LDS; the sorted tile is then written
-// out in index order, so each digit's run (16 keys on average for 4096 keys
-// over 256 digits) goes out as contiguous stores instead of one scattered
-// 8-byte store per key.
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
-    const uint64_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in, int64_t n, int shift,
-    const int64_t *__restrict__ hist_scan, int64_t nblocks, uint64_t *__restrict__ keys_out,
-    int32_t *__restrict__ vals_out)
+// Stable scatter: keys of a tile are ranked in tile order, round by round.
+__global__ void k_radix_scatter(const uint64_t *__restrict__ keys_in,
+                                const int32_t *__restrict__ vals_in, int64_t n, int shift,
+                                const int64_t *__restrict__ hist_scan, int64_t nblocks,
+                                uint64_t *__restrict__ keys_out, int32_t *__restrict__ vals_out)
 {
     constexpr int NW = kSortThreads / kWave;
     __shared__ int64_t base_off[kRadix];
-    __shared__ uint32_t tstart[kRadix];  // first slot of each digit in the sorted tile
     __shared__ uint32_t running[kRadix];
     __shared__ uint32_t wcnt[NW][kRadix];
-    __shared__ int64_t lw[NW];
-    __shared__ uint64_t lk[kSortTile];
-    __shared__ int32_t lv[kSortTile];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
-    // this tile's count of digit d: the next entry of the digit-major scan
-    // minus its own (the last entry of all is followed by n)
-    const int64_t hidx = (int64_t)threadIdx.x * nblocks + blockIdx.x;
-    const int64_t hcur = hist_scan[hidx];
-    const int64_t hnext = hidx + 1 < (int64_t)kRadix * nblocks ? hist_scan[hidx + 1] : n;
-    base_off[threadIdx.x] = hcur;
+    base_off[threadIdx.x] = hist_scan[(int64_t)threadIdx.x * nblocks + blockIdx.x];
     running[threadIdx.x] = 0;
-    int64_t excl = 0;
-    block_excl_scan(hnext - hcur, excl, lw);
-    tstart[threadIdx.x] = (uint32_t)excl;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    int64_t base = (int64_t)blockIdx.x * kSortTile;
     for (int r = 0; r < kSortRounds; ++r) {
 #pragma unroll
         for (int w = 0; w < NW; ++w) wcnt[w][threadIdx.x] = 0;
         __syncthreads();
-        const int64_t i = base + (int64_t)r * kSortThreads + threadIdx.x;
-        const bool valid = i < n;
-        const uint64_t k = valid ? keys_in[i] : 0;
-        const int32_t pv = (valid && vals_in) ? vals_in[i] : 0;
-        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        int64_t i = base + (int64_t)r * kSortThreads + threadIdx.x;
+        bool valid = i < n;
+        uint64_t k = valid ? keys_in[i] : 0;
+        int32_t pv = (valid && vals_in) ? vals_in[i] : 0;
+        uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
         // peers: lanes of this wave holding the same digit (valid only)
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const uint64_t bal = __ballot(valid && ((d >> b) & 1u));
+            uint64_t bal = __ballot(valid && ((d >> b) & 1u));
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
-        const uint32_t rank_w = (uint32_t)__popcll(peers & lt_mask);
-        const uint32_t cnt_w = (uint32_t)__popcll(peers);
+        uint32_t rank_w = (uint32_t)__popcll(peers & lt_mask);
+        uint32_t cnt_w = (uint32_t)__popcll(peers);
         if (valid && rank_w == 0) wcnt[wid][d] = cnt_w;
         __syncthreads();
         if (valid) {
             uint32_t before = 0;
             for (int w = 0; w < wid; ++w) before += wcnt[w][d];
-            const uint32_t slot = tstart[d] + running[d] + before + rank_w;
-            lk[slot] = k;
-            lv[slot] = pv;
+            int64_t pos = base_off[d] + running[d] + before + rank_w;
+            keys_out[pos] = k;
+            if (vals_out) vals_out[pos] = pv;
         }
         __syncthreads();
         uint32_t tot = 0;
@@ -248,14 +232,6 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         for (int w = 0; w < NW; ++w) tot += wcnt[w][threadIdx.x];
         running[threadIdx.x] += tot;
         __syncthreads();
-    }
-    const int32_t cnt = (int32_t)min<int64_t>(kSortTile, n - base);
-    for (int32_t idx = threadIdx.x; idx < cnt; idx += kSortThreads) {
-        const uint64_t k = lk[idx];
-        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
-        const int64_t pos = base_off[d] + (idx - (int32_t)tstart[d]);
-        keys_out[pos] = k;
-        if (vals_out) vals_out[pos] = lv[idx];
     }
 }
 
