@@ -657,7 +657,9 @@ __global__ void k_orient_transpose_hubs(
         const int64_t slot = k - (plus_prefix(cscan, mask, e) - plus_prefix(cscan, mask, rs));
         const int32_t a = rnk[pool[boff[v] + k]];
         const int64_t pa = pcoff[a], pn = pcoff[a + 1] - pa;
-        int64_t lo = 0, hi = pn;
+        // w is a hub and N+(a) is sorted by rank: w lies among its last
+        // nv - hub_lo entries (the hubs), so the search starts there
+        int64_t lo = max<int64_t>(0, pn - (nv - hub_lo)), hi = pn;
         while (lo < hi) {
             const int64_t mid = (lo + hi) >> 1;
             if (padj[pa + mid] < w) lo = mid + 1; else hi = mid;
