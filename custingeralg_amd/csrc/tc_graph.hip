@@ -429,7 +429,7 @@ void graph_destroy(Graph *g)
                       &g->coff, &g->crow, &g->work, &g->jobs, &g->tmp, &g->tmp2, &g->tmp3, &g->tmp4,
                       &g->accum, &g->bsrc, &g->bdst, &g->bflag, &g->bkeys, &g->bkeys2,
                       &g->bvals2, &g->boffb, &g->bdel, &g->flagpool, &g->vmark, &g->order,
-                      &g->rnk, &g->tasks, &g->pcnt, &g->hjobs};
+                      &g->rnk, &g->tasks, &g->pcnt, &g->hjobs, &g->outr};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf &b : g->sb) b.release();
     for (DevBuf &b : g->ov) b.release();

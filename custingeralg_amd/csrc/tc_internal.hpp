@@ -136,6 +136,7 @@ struct Graph {
     // ov = scratch of the orientation
     DevBuf order, rnk, tasks;
     DevBuf hjobs;  // the largest bin's jobs left to its hash-table launch
+    DevBuf outr;   // oriented per-vertex counts by rank, moved to vertex order after the count
     DevBuf ov[13];
     int64_t ntasks = 0;  // entries of the task array (oriented layout)
     int64_t nplus = 0;   // entries of the plus array (oriented layout)

@@ -888,11 +888,11 @@ __global__ void k_emit_jobs(const VInfo *__restrict__ vi, const int32_t *__restr
     }
 }
 
-// out[order[r]] += 2 * (sum of pcnt over N+(r)): the task-vertex share of the
+// outr[r] += 2 * (sum of pcnt over N+(r)): the task-vertex share of the
 // oriented count, one segmented wave sum per 64 entries of the plus array.
 __global__ void k_plus_row_sum(const int64_t *__restrict__ pcoff, const int32_t *__restrict__ pcrow,
-                               const uint32_t *__restrict__ pcnt, const int32_t *__restrict__ order,
-                               int32_t nv, int64_t nplus, uint32_t *__restrict__ out)
+                               const uint32_t *__restrict__ pcnt, int32_t nv, int64_t nplus,
+                               uint32_t *__restrict__ outr)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nchunks = ceil_div(nplus, kWave);
@@ -910,8 +910,18 @@ __global__ void k_plus_row_sum(const int64_t *__restrict__ pcoff, const int32_t 
         }
         const int32_t rn = __shfl_down(r, 1, kWave);
         const bool tail = lane == kWave - 1 || rn != r || e + 1 >= nplus;
-        if (e < nplus && tail && x) atomicAdd(&out[order[r]], 2u * x);
+        if (e < nplus && tail && x) atomicAdd(&outr[r], 2u * x);
     }
+}
+
+// out[order[r]] += outr[r]: the counts credited by rank added to the
+// per-vertex counts (every vertex once, after the count kernels).
+__global__ void k_rank_to_vertex(const int32_t *__restrict__ order, int32_t nv,
+                                 const uint32_t *__restrict__ outr, uint32_t *__restrict__ out)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x)
+        out[order[r]] += outr[r];
 }
 
 // gid[r] = graph holding vertex order[r] (or r): last g with voff[g] <= vertex.
@@ -937,9 +947,14 @@ struct CountArgs {
     const VInfo *vi;
     const TaskRec *tasks;  // oriented: task vertex and suffix of each task
     const int64_t *toff;   // oriented: first task of each owner
-    const int32_t *order;  // oriented: vertex of each rank (outputs are by vertex)
+    const int32_t *order;  // oriented: vertex of each rank
     uint32_t *pcnt;        // oriented: per N+ entry, the hits of its task
-    uint32_t *out;
+    uint32_t *out;         // per-vertex counts
+    // oriented: per-vertex counts by rank, credited by k_count_long and
+    // k_plus_row_sum without an order[] lookup and added to out at the end
+    // (k_count keeps crediting out: its hot counters, adjacent in rank order,
+    // would share cache lines there -- mid bin 20 -> 29 ms at R-MAT 24)
+    uint32_t *outr;
     unsigned long long *global;
     unsigned long long *strat;  // [3] tasks per strategy (may be null)
     const uint8_t *table;       // learned strategy table (policy LEARNED)
@@ -1705,16 +1720,16 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
         if (use_bm) {
             for (int i = threadIdx.x; i < L; i += nthr) {
                 const uint32_t ci = Cb[i];
-                if (ci) atomicAdd(&A.out[A.order[Sg[i]]], 2u * ci);
+                if (ci) atomicAdd(&A.outr[Sg[i]], 2u * ci);
             }
         } else {
             for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
                 const uint32_t ci = C[i];
-                if (ci) atomicAdd(&A.out[A.order[T.id_at(i)]], 2u * ci);
+                if (ci) atomicAdd(&A.outr[T.id_at(i)], 2u * ci);
             }
         }
         if (threadIdx.x == 0 && hjob) {
-            atomicAdd(&A.out[A.order[J.owner]], (uint32_t)(2 * hjob));
+            atomicAdd(&A.outr[J.owner], (uint32_t)(2 * hjob));
             if (A.per_graph) atomicAdd(&A.per_graph[A.gid[J.owner]], 6ull * hjob);
         }
         hits_all += (threadIdx.x == 0) ? hjob : 0ull;
@@ -2183,6 +2198,10 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
         hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nplus, 256, 65536)), dim3(256), 0, s,
                            g.pcnt.as<uint32_t>(), nplus, 0u);
         A.pcnt = g.pcnt.as<uint32_t>();
+        TC_TRY(g.outr.ensure((size_t)std::max<int32_t>(nv, 1) * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s,
+                           g.outr.as<uint32_t>(), (int64_t)nv, 0u);
+        A.outr = g.outr.as<uint32_t>();
     }
     if (oriented && jt[2] > 0) {
         TC_TRY(g.hjobs.ensure((size_t)jt[2] * sizeof(int64_t)));
@@ -2197,7 +2216,12 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     if (oriented && nplus > 0) {
         hipLaunchKernelGGL(k_plus_row_sum, dim3(grid_for(ceil_div(nplus, kWave), 4, 1 << 16)),
                            dim3(256), 0, s, g.ov[6].as<int64_t>(), g.ov[10].as<int32_t>(),
-                           g.pcnt.as<uint32_t>(), g.order.as<int32_t>(), nv, nplus, outp);
+                           g.pcnt.as<uint32_t>(), nv, nplus, g.outr.as<uint32_t>());
+        TC_HIP(hipGetLastError());
+    }
+    if (oriented) {
+        hipLaunchKernelGGL(k_rank_to_vertex, dim3(grid_for(nv, 256, 65536)), dim3(256), 0, s,
+                           g.order.as<int32_t>(), nv, g.outr.as<uint32_t>(), outp);
         TC_HIP(hipGetLastError());
     }
     TC_HIP(hipEventRecord(g.ev[7], s));
