@@ -48,6 +48,22 @@ sys.path.insert(0, ROOT)
 METRIC = "TC edges/sec + HBM GB/s at RMAT-24; learned vs fixed schedule, exact counts"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_count_long_rmat24_traffic.md")
+GOLDEN_FILE = os.path.join(ROOT, "tests", "golden", "big_fixtures.json")
+
+
+def golden(section: str):
+    """Expected results of a full-size workload, computed on the CPU by the
+    oracle (tests/golden/make_big_fixtures.py), or None."""
+    try:
+        with open(GOLDEN_FILE) as f:
+            return json.load(f).get(section)
+    except OSError:
+        return None
+
+
+def sha256(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
 def log(msg: str) -> None:
@@ -157,6 +173,20 @@ def bench_static(T, args) -> dict:
     Lm.unload()
     Lm.set_default_schedule(None)
 
+    # outside the timed region: the learned leg's output (the last one written
+    # to out) against the oracle's golden for this graph
+    check = "no golden for this graph"
+    gd = golden("rmat24")
+    if gd and (args.scale, args.ef, args.seed) == (gd["scale"], gd["edge_factor"], gd["seed"]):
+        pv = out.to_host()
+        if ref["glob"] != gd["global"] or sha256(pv) != gd["pv_sha256"]:
+            raise RuntimeError(f"count differs from the oracle's golden: global {ref['glob']} "
+                               f"!= {gd['global']} or per-vertex SHA-256 differs")
+        check = (f"global and per-vertex SHA-256 equal the oracle's golden "
+                 f"(T={gd['T']}, {gd['pv_wrapped']} per-vertex values wrapped past 2^32)")
+        del pv
+    log(f"[bench] parity: {check}")
+
     main = legs["learned"]
     st = main["stats"]
     # the dominant count kernel by event time: the small and mid bins' kernels,
@@ -188,6 +218,7 @@ def bench_static(T, args) -> dict:
                                f"(0.57,0.19,0.19,0.05) seed {args.seed}",
                    "nnz": nnz, "triangles": glob // 6, "global_count": glob,
                    "schedule": "autotuned + learned strategy table", "selector": source,
+                   "parity": check,
                    "legs_ms_per_step": {k: v["ms_per_step"] for k, v in legs.items()},
                    "learned_vs_fixed": legs["fixed"]["ms_per_step"] / main["ms_per_step"],
                    "tuned": {"small_max": tuned.small_max, "mid_max": tuned.mid_max,
@@ -206,52 +237,38 @@ def bench_static(T, args) -> dict:
     }
 
 
-def _stream_batch(T, g, nv, B, scale, seed, rng):
-    """80% inserts (half uniform, half R-MAT), 20% deletions of present edges;
-    pairs canonical (min,max), deduplicated, an insert equal to a delete dropped."""
-    n_ins = int(0.8 * B)
-    n_del = B - n_ins
-    nu = n_ins // 2
-    du, dv = T.DeviceArray(nu, np.int32), T.DeviceArray(nu, np.int32)
-    T.check(T.lib().tc_gen_er_pairs(nv, nu, seed, du.ptr, dv.ptr, None))
-    ru, rv = T.DeviceArray(n_ins - nu, np.int32), T.DeviceArray(n_ins - nu, np.int32)
-    T.check(T.lib().tc_gen_rmat_pairs(scale, n_ins - nu, 0.57, 0.19, 0.19, seed + 1, ru.ptr,
-                                      rv.ptr, None))
-    iu = np.concatenate([du.to_host(), ru.to_host()])
-    iv = np.concatenate([dv.to_host(), rv.to_host()])
-    off, adj = g.to_csr()
-    e = rng.integers(0, len(adj), n_del)
-    xu = (np.searchsorted(off, e, side="right") - 1).astype(np.int32)
-    xv = adj[e]
-    del off, adj
+def _device_pairs(T):
+    """The device pair generators in the form the workload recipe takes."""
+    def er(n, m, seed):
+        du, dv = T.DeviceArray(m, np.int32), T.DeviceArray(m, np.int32)
+        T.check(T.lib().tc_gen_er_pairs(n, m, seed, du.ptr, dv.ptr, None))
+        return du.to_host(), dv.to_host()
 
-    def canon(a, b):
-        lo, hi = np.minimum(a, b), np.maximum(a, b)
-        keep = lo != hi
-        return np.unique((lo[keep].astype(np.int64) << 32) | hi[keep].astype(np.int64))
-
-    ki, kd = canon(iu, iv), canon(xu, xv)
-    ki = ki[~np.isin(ki, kd)]
-    keys = np.concatenate([ki, kd])
-    u = (keys >> 32).astype(np.int32)
-    v = (keys & 0xFFFFFFFF).astype(np.int32)
-    dl = np.concatenate([np.zeros(len(ki), np.uint8), np.ones(len(kd), np.uint8)])
-    return u, v, dl
+    def rmat(scale, m, seed):
+        du, dv = T.DeviceArray(m, np.int32), T.DeviceArray(m, np.int32)
+        T.check(T.lib().tc_gen_rmat_pairs(scale, m, 0.57, 0.19, 0.19, seed, du.ptr, dv.ptr, None))
+        return du.to_host(), dv.to_host()
+    return er, rmat
 
 
 def bench_stream(T, args) -> dict:
-    scale, ef, seed = 22, 16, 3
+    from custingeralg_amd import workloads as W
+    scale, ef, seed = (W.STREAM_BASE[k] for k in ("scale", "edge_factor", "seed"))
     g = T.rmat_graph_device(scale, ef, seed)
     nv = g.nv
     log(f"[bench] stream base R-MAT {scale} ef {ef}: nnz={g.info()['nnz']}")
     _, total, _ = T.static_count(g, per_vertex=False)
     delta = T.DeviceArray(nv, np.uint32).zero()
-    rng = np.random.default_rng(seed)
-    sizes = [10 ** 3] * 5 + [10 ** 4] * 5 + [10 ** 5] * 5 + [10 ** 6] * 5
+    rng = np.random.default_rng(W.STREAM_RNG_SEED)
+    sizes = W.STREAM_SIZES
+    er, rmat = _device_pairs(T)
+    gd = golden("stream")
     per_size = {}
     t_all, n_all = 0.0, 0
     for i, B in enumerate(sizes):
-        u, v, dl = _stream_batch(T, g, nv, B, scale, 5000 + 7 * i, rng)
+        off, adj = g.to_csr()
+        u, v, dl = W.stream_batch(er, rmat, off, adj, nv, B, scale, W.stream_batch_seed(i), rng)
+        del off, adj
         du, dv, dd = (T.DeviceArray.from_host(x) for x in (u, v, dl))
         st = T.BatchStats()
         dg = C.c_int64(0)
@@ -265,6 +282,9 @@ def bench_stream(T, args) -> dict:
         _, recount, _ = T.static_count(g, per_vertex=False)
         if recount != total:
             raise RuntimeError(f"batch {i}: running total {total} != recount {recount}")
+        if gd and total != 6 * gd["batches"][i]["T"]:
+            raise RuntimeError(f"batch {i}: running total {total} != the oracle's "
+                               f"{6 * gd['batches'][i]['T']}")
         s = per_size.setdefault(B, {"pairs": 0, "s": 0.0, "ms_update": 0.0, "ms_delta": 0.0})
         s["pairs"] += len(u)
         s["s"] += dt
@@ -283,7 +303,10 @@ def bench_stream(T, args) -> dict:
             "higher_is_better": True, "dtype": "int32",
             "config": {"workload": f"streaming TC, R-MAT scale {scale} ef {ef} seed {seed} base, "
                                    "20 batches 5x10^3..5x10^6, 80% inserts / 20% deletes",
-                       "per_batch_size": rates, "final_global_count": total},
+                       "per_batch_size": rates, "final_global_count": total,
+                       "parity": ("running total equals a GPU recount and the oracle's T "
+                                  "after every batch" if gd else
+                                  "running total equals a GPU recount after every batch")},
             "roofline": None, "cpu_baseline": None}
 
 
@@ -307,12 +330,17 @@ def bench_batched(T, args) -> dict:
     ms, stats = timed_steps(T, step, args.steps, args.warmup)
     per_graph = pg.to_host()
     bg.close()
+    gd = golden("batched")
+    if gd and not np.array_equal(per_graph, np.array(gd["T"], np.uint64) * np.uint64(6)):
+        raise RuntimeError("per-graph counts differ from the oracle's golden")
     st = min(stats, key=lambda s: s["ms_total"])
     return {"metric": "batched TC edges/sec over 4096 graphs", "value": nnz / (ms * 1e-3),
             "unit": "edges/s", "ms_per_step": ms, "higher_is_better": True, "dtype": "int32",
             "config": {"workload": "batched static TC, 4096 R-MAT graphs, n log-uniform "
                                    "[1k,64k], ef U{4..32}, seeds 1000+i",
                        "nnz": nnz, "sum_per_graph": int(per_graph.sum()),
+                       "parity": ("every per-graph count equals the oracle's golden" if gd
+                                  else "no golden"),
                        "bins_ms": st["ms_bin"], "ms_prepare": st["ms_prepare"],
                        "bins": {"tasks": st["tasks_bin"], "jobs": st["jobs_bin"],
                                 "items": st["items_bin"], "stream_bytes": st["stream_bytes_bin"],

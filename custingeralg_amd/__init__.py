@@ -31,7 +31,7 @@ FMT_AUTO, FMT_DIMACS, FMT_SNAP, FMT_MTX = range(4)
 READ_AS_IS = 0x1
 TC_HOST, TC_DEVICE = 0, 1
 TC_GRAPH_ASSUME_SYMMETRIC, TC_GRAPH_NO_VALIDATE = 0x1, 0x2
-POLICY_FIXED, POLICY_LEARNED, POLICY_MERGE, POLICY_BSEARCH, POLICY_HASH = range(5)
+POLICY_FIXED, POLICY_LEARNED, POLICY_MERGE, POLICY_BSEARCH, POLICY_HASH, POLICY_MERGEPATH = range(6)
 METHOD_AUTO, METHOD_ORIENTED, METHOD_DIRECT = range(3)
 MLP_IN, MLP_HIDDEN, MLP_OUT = 4, 32, 3
 MLP_PARAMS = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN \
@@ -63,7 +63,8 @@ class Schedule(C.Structure):
     _fields_ = [("method", C.c_int32), ("policy", C.c_int32), ("small_max", C.c_int32),
                 ("mid_max", C.c_int32), ("job_items", C.c_int32), ("ratio_x16", C.c_int32),
                 ("persistent", C.c_int32), ("flush_items", C.c_int32),
-                ("hub_degree", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("hub_degree", C.c_int32), ("mpath_min", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):
@@ -74,7 +75,8 @@ class Stats(C.Structure):
                 ("method_used", C.c_int32), ("reserved0", C.c_int32),
                 ("items_bin", C.c_double * 3), ("stream_bytes_bin", C.c_double * 3),
                 ("ms_long", C.c_double * 2), ("tasks_long", C.c_int64 * 2),
-                ("items_long", C.c_double * 2), ("stream_bytes_long", C.c_double * 2)]
+                ("items_long", C.c_double * 2), ("stream_bytes_long", C.c_double * 2),
+                ("mergepath_tasks", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {
@@ -85,7 +87,8 @@ class Stats(C.Structure):
             "stream_bytes_bin": list(self.stream_bytes_bin),
             "ms_long": list(self.ms_long), "tasks_long": list(self.tasks_long),
             "items_long": list(self.items_long), "stream_bytes_long": list(self.stream_bytes_long),
-            "strategy_tasks": dict(zip(("merge", "bsearch", "hash"), self.strategy_tasks)),
+            "strategy_tasks": dict(zip(("merge", "bsearch", "hash"), self.strategy_tasks),
+                                   mergepath=int(self.mergepath_tasks)),
             "triangles": int(self.triangles), "method_used": int(self.method_used),
         }
 

@@ -29,7 +29,7 @@
 
 namespace tc {
 
-enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2 };
+enum { S_MERGE = 0, S_BSEARCH = 1, S_HASH = 2, S_MPATH = 3 };
 constexpr int kUnroll = 2;
 #ifndef TC_BIG_TASK
 #define TC_BIG_TASK 32
@@ -957,10 +957,12 @@ struct CountArgs {
     uint32_t *outr;
     unsigned long long *global;
     unsigned long long *strat;  // [3] tasks per strategy (may be null)
+    unsigned long long *strat_mp;  // tasks run as a merge path (may be null)
     const uint8_t *table;       // learned strategy table (policy LEARNED)
     int32_t table_dim;
     int32_t policy;
     int32_t ratio_x16;
+    int32_t mpath_min;   // fixed heuristic: balanced pairs of at least this length merge-path
     int32_t skip_lower;  // CSR convention: entries with dest < src are skipped
     // batched graphs: graph g owns vertices [gvoff[g], gvoff[g+1]); a job's
     // hits are added to per_graph[g] of its owner (triangles never span graphs)
@@ -995,6 +997,7 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
     case TC_POLICY_MERGE: return S_MERGE;
     case TC_POLICY_BSEARCH: return S_BSEARCH;
     case TC_POLICY_HASH: return S_HASH;
+    case TC_POLICY_MERGEPATH: return S_MPATH;
     case TC_POLICY_LEARNED:
         if (A.hash_only) return S_HASH;
         if (A.table) {
@@ -1012,6 +1015,7 @@ __device__ inline int choose_strategy(int32_t L, int32_t pl, const CountArgs &A)
     // streaming them through the table (lane-sequential walks in lockstep)
     int32_t sh = min(L, pl), lo = max(L, pl);
     if (A.ratio_x16 > 0 && (int64_t)lo * 16 <= (int64_t)sh * A.ratio_x16 && lo <= 96) return S_MERGE;
+    if (A.mpath_min > 0 && sh >= A.mpath_min && lo <= 2 * sh) return S_MPATH;
     // searching the owner list into a much longer task list beats streaming it
     if ((int64_t)L * lg_bucket(pl) * 4 < (int64_t)pl) return S_BSEARCH;
     return S_HASH;
@@ -1105,6 +1109,49 @@ __device__ inline uint32_t stream_tasks(const int32_t *adj, int32_t pl, int64_t 
     return mine_h;
 }
 
+// |X cap Y| of two strictly increasing lists by the whole wavefront, as a
+// merge path over 64-entry windows (the cooperative count of
+// singleIntersection / intersectPath, src/static_triangle_counting/cct.cu:
+// 145-199, with the diagonal search of bSearchIP, :91-132).  Each step loads
+// the next 64 entries of both lists (coalesced) and every lane finds the
+// co-rank of its x in the Y window -- the merge-path split at x's diagonal --
+// by a six-step binary search over the window's lanes.  Only values up to
+// m = min(last x, last y) are settled by the two windows: a pair x == y <= m
+// has both ends in them, so it is counted there and nowhere else (no lane
+// hands a match at a split to its neighbour, unlike fixStartPointIP's
+// firstFound flags, :134-143 and :181-196).  Both windows then advance past
+// their entries <= m, at least one of them by its whole length.  hit(x, r) is
+// called by the lane of every match, r being x's index in the Y window.
+// Called by the whole wave with wave-uniform X, nx, Y, ny.
+template <class Hit>
+__device__ inline uint32_t merge_path_count(const int32_t *__restrict__ X, int32_t nx,
+                                            const int32_t *__restrict__ Y, int32_t ny, Hit &&hit)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    int32_t ix = 0, iy = 0;
+    uint32_t h = 0;
+    while (ix < nx && iy < ny) {
+        const int32_t cx = min(kWave, nx - ix), cy = min(kWave, ny - iy);
+        const int32_t x = lane < cx ? X[ix + lane] : INT32_MAX;
+        const int32_t y = lane < cy ? Y[iy + lane] : INT32_MAX;
+        const int32_t m = min(__builtin_amdgcn_readlane(x, cx - 1), __builtin_amdgcn_readlane(y, cy - 1));
+        // r = number of window entries y < x among lanes 0..62 (lane 63 is
+        // never probed: if it alone is below x, x is not in the window)
+        int32_t r = 0;
+#pragma unroll
+        for (int st = kWave / 2; st > 0; st >>= 1) {
+            const int32_t p = __shfl(y, r + st - 1, kWave);
+            r += (p < x) ? st : 0;
+        }
+        const bool match = x <= m && __shfl(y, r, kWave) == x;
+        h += (uint32_t)__popcll(__ballot(match));
+        if (match) hit(x, iy + r);
+        ix += __popcll(__ballot(x <= m));
+        iy += __popcll(__ballot(y <= m));
+    }
+    return h;
+}
+
 // The part of a task list [pb, pb+pl) inside an owner tile's id range
 // [lo_v, hi_v] (tiled owners): two binary searches.
 __device__ inline void tile_filter(const int32_t *__restrict__ adj, int64_t &pb, int32_t &pl,
@@ -1158,7 +1205,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
     const int wid = threadIdx.x / kWave;
     const int nthr = NW * kWave;
     unsigned long long hits_all = 0;
-    unsigned long long sc[3] = {0, 0, 0};
+    unsigned long long sc[4] = {0, 0, 0, 0};
 
     // a fixed round-robin deal of jobs: a device counter per job (as in
     // k_count_long's bitmap launch) serialises on one address at these jobs'
@@ -1198,7 +1245,15 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             }
             if (!__syncthreads_or(other)) continue;
         }
-        if (use_bm) {
+        // under the merge-path policy every task walks the sorted tile itself:
+        // no table, and third-vertex counters indexed by tile position
+        const bool mp_only = A.policy == TC_POLICY_MERGEPATH;
+        const bool by_pos = use_bm || mp_only;
+        if (mp_only) {
+            if (ORIENTED)
+                for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
+            __syncthreads();
+        } else if (use_bm) {
             for (int i = threadIdx.x; i <= nwords; i += nthr) BMw[i] = make_uint2(0u, 0u);
             if (ORIENTED)
                 for (int i = threadIdx.x; i < L; i += nthr) C[i] = 0;
@@ -1305,6 +1360,17 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
             tcnt[wid][lane] = 0;
             uint32_t mcount = 0;
             wave_sync();
+
+            // -- merge path: the whole wave on one task at a time
+            for (uint64_t mp = __ballot(strat == S_MPATH); mp; mp &= mp - 1) {
+                const int jm = __ffsll((unsigned long long)mp) - 1;
+                const int32_t *X = A.adj + readlane_i64(pb, jm);
+                const int32_t nx = __builtin_amdgcn_readlane(pl, jm);
+                const uint32_t h = merge_path_count(X, nx, Sg, L, [&](int32_t x, int32_t pos) {
+                    if (ORIENTED) atomicAdd(&C[by_pos ? pos : lookup(x)], 1u);
+                });
+                if (lane == jm) tcnt[wid][jm] = h;
+            }
 
             // Long hash tasks are streamed in 64-entry chunks across the whole
             // wave (stream_tasks): one batch of kBigUnroll loads per lane spans
@@ -1508,7 +1574,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
         unsigned long long hjob = 0;
         for (int w = 0; w < NW; ++w) hjob += red[w];
         if (ORIENTED) {
-            if (use_bm) {
+            if (by_pos) {
                 for (int i = threadIdx.x; i < L; i += nthr) {
                     const uint32_t ci = C[i];
                     if (ci) atomicAdd(&A.out[A.order[Sg[i]]], 2u * ci);
@@ -1531,9 +1597,9 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const
     if (threadIdx.x == 0 && hits_all) atomicAdd(A.global, (ORIENTED ? 6ull : 1ull) * hits_all);
     if (A.strat) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < 4; ++q) {
             unsigned long long s = wave_sum<unsigned long long>(sc[q]);
-            if (lane == 0 && s) atomicAdd(&A.strat[q], s);
+            if (lane == 0 && s) atomicAdd(q < 3 ? &A.strat[q] : A.strat_mp, s);
         }
     }
 }
@@ -1804,15 +1870,19 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
             Al.hash_only = 1;
             CountArgs As = Al;
             As.long_external = 1;
-            const bool forced = A.policy == TC_POLICY_MERGE || A.policy == TC_POLICY_BSEARCH;
+            // (forced non-hash policies leave k_count_long nothing to stream)
+            const bool forced = A.policy == TC_POLICY_MERGE || A.policy == TC_POLICY_BSEARCH ||
+                                A.policy == TC_POLICY_MERGEPATH;
             if (forced)
                 hipLaunchKernelGGL((k_count<kL2, kNW2, true>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, As);
             const int64_t gb = std::min<int64_t>(njobs, (int64_t)cus * 3);
             const int64_t gh = std::min<int64_t>(njobs, (int64_t)cus * 2);
             TC_HIP(hipEventRecord(ev_long[0], s));
-            hipLaunchKernelGGL((k_count_long<kL2, kNW2, true>), dim3((unsigned)gb), dim3(kNW2 * kWave), 0, s, jobs, njobs, Al);
+            if (!forced)
+                hipLaunchKernelGGL((k_count_long<kL2, kNW2, true>), dim3((unsigned)gb), dim3(kNW2 * kWave), 0, s, jobs, njobs, Al);
             TC_HIP(hipEventRecord(ev_long[1], s));
-            hipLaunchKernelGGL((k_count_long<kL2, kNW2, false>), dim3((unsigned)gh), dim3(kNW2 * kWave), 0, s, jobs, njobs, Al);
+            if (!forced)
+                hipLaunchKernelGGL((k_count_long<kL2, kNW2, false>), dim3((unsigned)gh), dim3(kNW2 * kWave), 0, s, jobs, njobs, Al);
             TC_HIP(hipEventRecord(ev_long[2], s));
         } else
             hipLaunchKernelGGL((k_count<kL2, kNW2, false>), dim3((unsigned)grid), dim3(kNW2 * kWave), 0, s, jobs, njobs, A);
@@ -2057,7 +2127,8 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
 
     // scratch: accum[0] = global, [1..3] = strategy counts, [4..9] bin tasks
     // and bytes, [10..12] items, [13..15] tile items, [16..21] long-task stats,
-    // [22..24] job counters of the largest bin's two launches
+    // [22..24] job counters of the largest bin's two launches, [25] merge-path
+    // tasks
     TC_TRY(g.accum.ensure(32 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
     uint32_t *outp = out;
@@ -2169,10 +2240,12 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     A.out = outp;
     A.global = acc;
     A.strat = acc + 1;
+    A.strat_mp = acc + 25;
     A.long_stats = acc + 16;
     A.job_ctr = acc + 22;
     A.policy = sched.policy;
     A.ratio_x16 = std::max(sched.ratio_x16, 0);
+    A.mpath_min = std::max(sched.mpath_min, 0);
     A.skip_lower = skip_lower ? 1 : 0;
     A.per_graph = g.batch_per_graph;
     A.gid = nullptr;
@@ -2227,7 +2300,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
     TC_HIP(hipEventRecord(g.ev[7], s));
     TC_HIP(hipEventSynchronize(g.ev[7]));
 
-    unsigned long long hacc[24];
+    unsigned long long hacc[26];
     TC_HIP(hipMemcpy(hacc, acc, sizeof(hacc), hipMemcpyDeviceToHost));
     if (global) *global = hacc[0];
     if (stats) {
@@ -2249,6 +2322,7 @@ int static_count(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &s
                                      4.0 * (double)hacc[13 + b];
         }
         for (int q = 0; q < 3; ++q) st.strategy_tasks[q] = (int64_t)hacc[1 + q];
+        st.mergepath_tasks = (int64_t)hacc[25];
         if (oriented && jt[2] > 0) {
             for (int k = 0; k < 2; ++k) {
                 (void)hipEventElapsedTime(&ms, g.ev[8 + k], g.ev[9 + k]);

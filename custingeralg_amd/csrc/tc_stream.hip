@@ -60,15 +60,22 @@ __global__ void k_batch_check(const int32_t *__restrict__ u, const int32_t *__re
 }
 
 // key = (min << (bv+1)) | (max << 1) | is_delete
+// Sort keys (pair, op) of the raw batch; ndel[0] (zeroed) receives the number
+// of deletion requests, in-batch repeats included.
 __global__ void k_batch_keys(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
                              const uint8_t *__restrict__ del, int64_t B, int bv,
-                             uint64_t *__restrict__ keys)
+                             uint64_t *__restrict__ keys, unsigned long long *__restrict__ ndel)
 {
+    unsigned long long nd = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B;
          i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t a = (uint32_t)min(u[i], v[i]), b = (uint32_t)max(u[i], v[i]);
-        keys[i] = (a << (bv + 1)) | (b << 1) | (uint64_t)((del && del[i]) ? 1 : 0);
+        const bool d = del && del[i];
+        keys[i] = (a << (bv + 1)) | (b << 1) | (uint64_t)(d ? 1 : 0);
+        nd += d;
     }
+    const unsigned long long part[1] = {nd};
+    block_add_u64<1>(part, ndel);
 }
 
 __global__ void k_unique_split(const uint64_t *__restrict__ keys, int64_t n,
@@ -896,8 +903,9 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     // sort (pair, op) keys and split unique inserts / deletes
     uint64_t *keys = buf<uint64_t>(g, 0, B);
     TC_NEED(keys);
+    // acc[5]: deletion requests of the raw batch
     hipLaunchKernelGGL(k_batch_keys, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, u, v, is_del,
-                       B, bv, keys);
+                       B, bv, keys, acc + 5);
     TC_TRY(radix_sort_u64(keys, nullptr, B, bits, g.sb[1], g.sb[2], g.tmp, s));
     int32_t *fi = buf<int32_t>(g, 3, B), *fd = buf<int32_t>(g, 4, B);
     int64_t *pi = buf<int64_t>(g, 5, B + 1), *pd = buf<int64_t>(g, 6, B + 1);
@@ -965,15 +973,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_NEED(cnt);
         TC_TRY(scan_counts_i32(pres, cnt, nD, g.tmp, s));
         bs.deleted = read_i64(cnt + nD, s);
-        bs.missing_deletes = (int64_t)0;
         g.st.nnz -= 2 * bs.deleted;
-    }
-    // duplicates: deletes of absent pairs plus in-batch repeats
-    {
-        int64_t nDel = 0;
-        // count delete requests in the raw batch from the op bit of the sorted keys
-        // (unique deletes = nD); repeats = requests - unique
-        (void)nDel;
     }
 
     // ---------------------------------------------------------- insertions
@@ -989,7 +989,6 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_TRY(scan_counts_i32(absent, sidx, nI, g.tmp, s));
         const int64_t nNew = read_i64(sidx + nI, s);
         bs.inserted = nNew;
-        bs.dup_inserts = nI - nNew;
         if (nNew > 0) {
             const int64_t nd = 2 * nNew;
             uint64_t *dk = buf<uint64_t>(g, 18, nd);
@@ -1059,12 +1058,13 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     (void)hipEventElapsedTime(&ms, g.ev[8], g.ev[13]);
     bs.ms_delta = ms_delta;
     bs.ms_update = ms - ms_delta;
-    bs.missing_deletes = nD - bs.deleted;
-    if (count && dglobal) {
-        unsigned long long h = 0;
-        TC_HIP(hipMemcpy(&h, acc + 2, sizeof(h), hipMemcpyDeviceToHost));
-        *dglobal = (int64_t)h;
-    }
+    // no-ops: requests (in-batch repeats included) that changed nothing
+    unsigned long long h[4] = {0, 0, 0, 0};
+    TC_HIP(hipMemcpy(h, acc + 2, sizeof(h), hipMemcpyDeviceToHost));
+    const int64_t del_requests = (int64_t)h[3];
+    bs.missing_deletes = del_requests - bs.deleted;
+    bs.dup_inserts = (B - del_requests) - bs.inserted;
+    if (count && dglobal) *dglobal = (int64_t)h[0];
     g.degree_stale = true;
     if (stats) *stats = bs;
     return TC_OK;

@@ -101,6 +101,10 @@ int tc_graph_degrees(tc_graph *g, int32_t *deg_dev);
 #define TC_POLICY_MERGE   2   /* force one strategy (tests, timing)        */
 #define TC_POLICY_BSEARCH 3
 #define TC_POLICY_HASH    4
+/* coalesced merge path: one wavefront splits one pair by merge-path
+ * diagonals (singleIntersection / workPerThreadIP / bSearchIP /
+ * fixStartPointIP / intersectPath, src/static_triangle_counting/cct.cu:54-199) */
+#define TC_POLICY_MERGEPATH 5
 
 /* Counting method. */
 #define TC_METHOD_AUTO     0  /* oriented if symmetric and loop-free, else direct */
@@ -120,7 +124,10 @@ typedef struct {
                            /*   flush of third-vertex counters)                      */
     int32_t hub_degree;    /* oriented: owners of at least this degree build their   */
                            /*   task lists from their own rows (0: default 65536)    */
-    int32_t reserved[7];
+    int32_t mpath_min;     /* fixed heuristic: balanced pairs (long <= 2 * short)     */
+                           /*   whose shorter list has at least this many entries    */
+                           /*   run as a merge path (0, the default: never)          */
+    int32_t reserved[6];
 } tc_schedule;
 
 /* Fill *s with the defaults (autotuned table if one was loaded). */
@@ -150,6 +157,7 @@ typedef struct {
     int64_t tasks_long[2];
     double items_long[2];
     double stream_bytes_long[2];
+    int64_t mergepath_tasks; /* tasks run as a wavefront-wide merge path     */
 } tc_stats;
 
 /* ------------------------------------------------------ static count -- */

@@ -268,6 +268,51 @@ def apply_batch_csr(n, off, adj, ins_u=(), ins_v=(), del_u=(), del_v=()):
     return csr_from_lists([sorted(l) for l in lists])
 
 
+def apply_batch_sorted(n, off, adj, u, v, is_delete):
+    """apply_batch_csr on sorted int64 keys (row << 32 | column), for graphs
+    too large for Python sets: removes the present pairs of the deletions,
+    then adds the absent pairs of the insertions (self-loops ignored), both
+    directions of each.  Returns (off, adj, inserted, deleted) with the
+    numbers of undirected pairs actually inserted and deleted."""
+    off = np.asarray(off, np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    keys = (rows << 32) | np.asarray(adj, np.int64)
+    del rows
+    u = np.asarray(u, np.int64)
+    v = np.asarray(v, np.int64)
+    d = np.asarray(is_delete, bool) if is_delete is not None else np.zeros(len(u), bool)
+
+    def both(a, b):
+        keep = a != b
+        a, b = a[keep], b[keep]
+        return np.unique(np.concatenate([(a << 32) | b, (b << 32) | a]))
+
+    def present(k):
+        if len(keys) == 0 or len(k) == 0:
+            return np.zeros(len(k), bool), np.zeros(len(k), np.int64)
+        pos = np.searchsorted(keys, k)
+        hit = pos < len(keys)
+        hit[hit] = keys[pos[hit]] == k[hit]
+        return hit, pos
+
+    dk = both(u[d], v[d])
+    hit, pos = present(dk)
+    if hit.any():
+        keep = np.ones(len(keys), bool)
+        keep[pos[hit]] = False
+        keys = keys[keep]
+    deleted = int(hit.sum()) // 2
+    ik = both(u[~d], v[~d])
+    hit, _ = present(ik)
+    ik = ik[~hit]
+    if len(ik):
+        keys = np.insert(keys, np.searchsorted(keys, ik), ik)
+    inserted = len(ik) // 2
+    new_off = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(keys >> 32, minlength=n), out=new_off[1:])
+    return new_off, (keys & 0xFFFFFFFF).astype(np.int32), inserted, deleted
+
+
 # -------------------------------------------------------------- load balance
 def lb_partition(work):
     work = np.ascontiguousarray(work, np.int64)

@@ -21,3 +21,15 @@ def _print_case(item):
 @pytest.hookimpl(tryfirst=True)
 def pytest_runtest_setup(item):
     _print_case(item)
+
+
+@pytest.fixture(autouse=True)
+def _restore_process_defaults(request):
+    """A GPU test that autotunes or loads a strategy table changes process-wide
+    defaults (the default schedule, the learned table); every test after it
+    starts from the built-in ones again."""
+    yield
+    if request.node.get_closest_marker("gpu"):
+        from custingeralg_amd import learned
+        learned.set_default_schedule(None)
+        learned.unload()

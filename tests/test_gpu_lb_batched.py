@@ -72,7 +72,8 @@ def test_batched_handle_repeated_counts():
     pg = T.DeviceArray(len(n), np.uint64)
     pv = T.DeviceArray(int(voff_h[-1]), np.uint32)
     for sched in (None, T.default_schedule(small_max=32, mid_max=256, job_items=1024),
-                  T.default_schedule(policy=T.POLICY_MERGE)):
+                  T.default_schedule(policy=T.POLICY_MERGE),
+                  T.default_schedule(policy=T.POLICY_MERGEPATH)):
         bg.count(pg, pv, sched)
         assert np.array_equal(pg.to_host(), want)
         assert np.array_equal(pv.to_host(), want_pv)

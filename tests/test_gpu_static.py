@@ -13,7 +13,7 @@ from tests import graphs as GR
 
 pytestmark = pytest.mark.gpu
 
-POLICIES = [T.POLICY_FIXED, T.POLICY_MERGE, T.POLICY_BSEARCH, T.POLICY_HASH]
+POLICIES = [T.POLICY_FIXED, T.POLICY_MERGE, T.POLICY_BSEARCH, T.POLICY_HASH, T.POLICY_MERGEPATH]
 
 
 def count(off, adj, **sched):
@@ -111,11 +111,48 @@ def test_rmat18_schedules_identical():
     variants = [dict(), dict(small_max=16, mid_max=128, job_items=1024),
                 dict(small_max=64, mid_max=512, job_items=65536), dict(policy=T.POLICY_HASH),
                 dict(policy=T.POLICY_BSEARCH), dict(policy=T.POLICY_MERGE),
+                dict(policy=T.POLICY_MERGEPATH),
+                dict(mpath_min=4), dict(mpath_min=48),  # balanced pairs merge-path
                 dict(ratio_x16=64)]  # the fixed heuristic with its merges on
     for v in variants:
         pv, glob, _ = count(off, adj, **v)
         assert np.array_equal(pv, want), v
         assert glob == 6 * Tt
+
+
+@pytest.mark.parametrize("mpath_min", [1, 16, 200])
+@pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
+def test_mixed_merge_path_tasks(mpath_min, method):
+    """The fixed heuristic sends balanced pairs to the wavefront merge path
+    while the other tasks of the same jobs probe the LDS table: third-vertex
+    hits of merge-path tasks then take their slot from the table."""
+    off, adj = O.rmat_graph(15, 24, 9)
+    ref = O.custinger_count(off, adj)
+    pv, glob, st = count(off, adj, method=method, mpath_min=mpath_min)
+    assert np.array_equal(pv, ref)
+    assert glob == int(ref.astype(np.uint64).sum())
+    assert st["strategy_tasks"]["mergepath"] > 0
+    assert sum(st["strategy_tasks"][k] for k in ("hash", "bsearch")) > 0
+
+
+@pytest.mark.parametrize("n", [63, 64, 65, 127, 128, 129])
+def test_merge_path_window_boundaries(n):
+    """K_n and K_n minus a perfect matching put list lengths on and around the
+    64-entry windows of the merge path."""
+    off, adj, t = GR.complete_fast(n)
+    for method in (T.METHOD_AUTO, T.METHOD_DIRECT):
+        pv, glob, st = count(off, adj, method=method, policy=T.POLICY_MERGEPATH)
+        assert np.all(pv == (2 * t).astype(np.uint32)), method
+        assert st["strategy_tasks"]["mergepath"] > 0
+    keep = np.ones(len(adj), bool)
+    rows = np.repeat(np.arange(n), np.diff(off))
+    keep[(adj == (rows ^ 1))] = False  # drop the edges {2i, 2i+1}
+    off2 = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=n))]).astype(np.int64)
+    adj2 = adj[keep]
+    ref = O.custinger_count(off2, adj2)
+    for method in (T.METHOD_AUTO, T.METHOD_DIRECT):
+        pv, glob, _ = count(off2, adj2, method=method, policy=T.POLICY_MERGEPATH)
+        assert np.array_equal(pv, ref), method
 
 
 @pytest.mark.parametrize("n", [130, 300, 600])

@@ -104,11 +104,20 @@ def test_duplicates_and_noops():
     want_off, want_adj = O.apply_batch_csr(n, off, adj, ins_u=u, ins_v=v)
     got = g.to_csr()
     assert np.array_equal(got[0], want_off) and np.array_equal(got[1], want_adj)
-    assert st["dup_inserts"] >= 5
-    # deleting an absent pair is a no-op
-    dg, st = T.batch_apply(g, np.array([0], np.int32), np.array([n - 1], np.int32),
-                           np.ones(1, np.uint8))
-    assert st["deleted"] + st["missing_deletes"] == 1
+    # every request that changed nothing is a no-op, in-batch repeats included
+    added = (len(want_adj) - len(adj)) // 2
+    assert st["inserted"] == added
+    assert st["dup_inserts"] == len(u) - added
+    assert st["deleted"] == 0 and st["missing_deletes"] == 0
+    # deletions: an absent pair twice, a present pair three times
+    cur_off, cur_adj = got
+    assert not np.isin(n - 1, cur_adj[cur_off[0]:cur_off[1]])
+    pu, pv = int(present_u[0]), int(present_v[0])
+    du = np.array([0, 0, pu, pv, pu], np.int32)
+    dv = np.array([n - 1, n - 1, pv, pu, pv], np.int32)
+    dg, st = T.batch_apply(g, du, dv, np.ones(5, np.uint8))
+    assert st["deleted"] == 1 and st["missing_deletes"] == 4
+    assert st["inserted"] == 0 and st["dup_inserts"] == 0
     g.close()
 
 

@@ -249,3 +249,115 @@ def test_lb_partition_properties(n, avg):
         needle = int(prefix[n] * (b / P))
         # lower bound over prefix[0 .. n-1)
         assert p == np.searchsorted(prefix[:n - 1], needle, side="left")
+
+
+# ------------------------------------------------- CSR variant, per vertex
+def _csr_variant_by_triangles(n, off, adj):
+    """count_all_trianglesGPU's per-vertex value from an explicit triangle
+    list (tests/cct_csr.cu:245-275): the row of src meets each triangle
+    {src, dest, w} once per dest > src, so a triangle a < b < c adds 2 to a,
+    1 to b and 0 to c."""
+    from itertools import combinations
+    nb = [set(adj[off[v]:off[v + 1]].tolist()) for v in range(n)]
+    out = np.zeros(n, np.int64)
+    for a, b, c in combinations(range(n), 3):
+        if b in nb[a] and c in nb[a] and c in nb[b]:
+            out[a] += 2
+            out[b] += 1
+    return out.astype(np.uint32)
+
+
+def test_csr_count_k4_per_vertex():
+    off, adj, _ = GR.complete(4)
+    assert O.csr_count(off, adj).tolist() == [6, 4, 2, 0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_csr_count_per_vertex_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(5, 24))
+    p = [0.2, 0.5, 0.9][seed % 3]
+    e = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < p]
+    off, adj = O.csr_from_edges(n, e)
+    assert np.array_equal(O.csr_count(off, adj), _csr_variant_by_triangles(n, off, adj))
+
+
+@pytest.mark.parametrize("name", ["W8", "F5", "K10", "K7_9", "grid12x9"])
+def test_csr_count_per_vertex_closed_forms(name):
+    off, adj, _ = GR.closed_form_cases()[name]
+    n = len(off) - 1
+    assert np.array_equal(O.csr_count(off, adj), _csr_variant_by_triangles(n, off, adj))
+
+
+# ----------------------------------------- load balance: hand-derived cases
+# LoadBalanceArray (include/load_balance.cuh:77-102): P = 200 / 500 / 1000 by
+# average work (< 4, < 7, else), at most len; needles total*(b/P) in double
+# (src/load_balance.cu:57-61); lower bound over prefix[0 .. len-1).
+@pytest.mark.parametrize("work,parts", [
+    # len < every P: P = len; unit work, needles 0..9
+    ([1] * 10, list(range(10))),
+    # one heavy vertex: prefix 0,1,2,3,94..100; needles 0,10,...,90 all land on 4
+    ([1, 1, 1, 91, 1, 1, 1, 1, 1, 1], [0] + [4] * 9),
+    # heavy last vertex: haystack prefix[0..999) = 0..998, needles 101b;
+    # needles past 998 give len-1 = 999 (no entry of the haystack reaches
+    # them).  The fp64 needle 101000 * (9/1000.0) is 908.99999... and
+    # truncates to 908
+    ([1] * 999 + [100001], [101 * b for b in range(9)] + [908] + [999] * 990),
+])
+def test_lb_partition_hand_cases(work, parts):
+    _, got = O.lb_partition(np.array(work, np.int64))
+    assert got.tolist() == parts
+
+
+@pytest.mark.parametrize("w,P,step", [
+    (1, 200, 25),    # avg 1 < 4: P = 200, needles 25b, parts 25b
+    (3, 200, 25),    # avg 3
+    (4, 500, 10),    # avg 4 is not < 4: P = 500, needles 40b over prefix 4i
+    (5, 500, 10),    # avg 5: P = 500, needles 50b over prefix 5i
+    (6, 500, 10),
+    (7, 1000, 5),    # avg 7 is not < 7: P = 1000
+    (10, 1000, 5),
+])
+def test_lb_partition_uniform_work(w, P, step):
+    _, got = O.lb_partition(np.full(5000, w, np.int64))
+    assert len(got) == P
+    want = [step * b for b in range(P)]
+    if w == 1:
+        # unit work: the needle is the partition point itself, and for these
+        # b the fp64 product 5000 * (b / 200.0) falls just below 25b (e.g.
+        # 1424.9999999999998 for b = 57), so the truncated needle is 25b - 1
+        for b in (57, 69, 113, 114, 138, 139, 163):
+            want[b] -= 1
+    assert got.tolist() == want
+
+
+def test_lb_partition_float_average():
+    # 3999 work over 1000 entries: avg 3.999 < 4 -> P = 200
+    work = np.full(1000, 4, np.int64)
+    work[0] = 3
+    _, got = O.lb_partition(work)
+    assert len(got) == 200
+
+
+# -------------------------------------------- fast store update (oracle)
+@pytest.mark.parametrize("seed", range(4))
+def test_apply_batch_sorted_matches_sets(seed):
+    from custingeralg_amd import workloads as W
+    off, adj = O.rmat_graph(9, 8, 20 + seed)
+    n = len(off) - 1
+    rng = np.random.default_rng(seed)
+    for i in range(3):
+        u, v, dl = W.stream_batch(O.er_pairs, O.rmat_pairs, off, adj, n, 400, 9,
+                                  W.stream_batch_seed(i), rng)
+        n_del = int(dl.sum())
+        # in-batch repeats and a self-loop, which both updates ignore
+        u = np.concatenate([u, u[:5], [3]]).astype(np.int32)
+        v = np.concatenate([v, v[:5], [3]]).astype(np.int32)
+        dl = np.concatenate([dl, dl[:5], [0]]).astype(np.uint8)
+        got = O.apply_batch_sorted(n, off, adj, u, v, dl)
+        d = dl.astype(bool)
+        want = O.apply_batch_csr(n, off, adj, ins_u=u[~d], ins_v=v[~d], del_u=u[d], del_v=v[d])
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        assert got[3] == n_del  # every drawn deletion is present
+        assert len(got[1]) == len(adj) + 2 * (got[2] - got[3])
+        off, adj = got[0], got[1]
