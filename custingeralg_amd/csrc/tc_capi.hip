@@ -234,7 +234,7 @@ static int check_schedule(const tc_schedule *s)
     if (!s) return TC_OK;
     TC_CHECK_ARG(s->method >= TC_METHOD_AUTO && s->method <= TC_METHOD_DIRECT,
                  "tc_schedule: bad method");
-    TC_CHECK_ARG(s->policy >= TC_POLICY_FIXED && s->policy <= TC_POLICY_HASH,
+    TC_CHECK_ARG(s->policy >= TC_POLICY_FIXED && s->policy <= TC_POLICY_MERGEPATH,
                  "tc_schedule: bad policy");
     TC_CHECK_ARG(s->small_max >= 1 && s->mid_max >= s->small_max && s->job_items >= 1,
                  "tc_schedule: bad bin thresholds");

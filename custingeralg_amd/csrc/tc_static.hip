@@ -1143,7 +1143,10 @@ __device__ inline uint32_t merge_path_count(const int32_t *__restrict__ X, int32
             const int32_t p = __shfl(y, r + st - 1, kWave);
             r += (p < x) ? st : 0;
         }
-        const bool match = x <= m && __shfl(y, r, kWave) == x;
+        // the shuffle runs on every lane: a lane that ds_bpermute reads from
+        // must be active, or the read returns 0
+        const int32_t yr = __shfl(y, r, kWave);
+        const bool match = x <= m && yr == x;
         h += (uint32_t)__popcll(__ballot(match));
         if (match) hit(x, iy + r);
         ix += __popcll(__ballot(x <= m));
