@@ -21,14 +21,16 @@ hash-task kernel on bitmap or hash-table owner tiles.
 Other workloads (``--workload``): ``stream`` (R-MAT scale 22 ef 16 seed 3 base,
 20 update batches of 10^3..10^6 pairs, updates/s, running total checked against
 a recount after every batch), ``batched`` (4096 R-MAT graphs of 1k-64k
-vertices counted in one call, edges/s) and ``copy`` (a plain device copy of
-known size, the calibration kernel for the HBM traffic counters).
+vertices counted in one call, edges/s), ``copy`` (a plain device copy of
+known size, the calibration kernel for the HBM traffic counters) and ``cpu``
+(the reference CPU path counting ER-16 and R-MAT 20 in full, on one and on
+all host cores; no GPU).
 
 The CPU baseline times the oracle's restatement of the reference CPU path
 (hostCountTriangles' per-entry merge, tests/cct.cpp:32-89) on a uniform
 sample of the same graph's entries, on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload static|stream|batched|copy]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload static|stream|batched|copy|cpu]
 """
 from __future__ import annotations
 
@@ -90,9 +92,45 @@ def cpu_baseline(off, adj, budget_s: float, seed: int = 7) -> dict:
         if dt < budget_s / 4:
             n *= 2
     return {"value": done / t_all, "unit": "edges/s", "cores": threads, "kind": "port",
-            "sample": f"{done} directed entries drawn uniformly from the same graph; each runs "
-                      f"the per-entry merge of hostCountTriangles (tests/cct.cpp:32-89); "
-                      f"{t_all:.1f} s on {threads} threads"}
+            "cpu_model": cpu_model(),
+            "sample": f"sampled, not a full count: {done} directed entries drawn uniformly from "
+                      f"the same graph ({done / nnz:.2%} of its {nnz}); each runs the per-entry "
+                      f"merge of hostCountTriangles (tests/cct.cpp:32-89); {t_all:.1f} s on "
+                      f"{threads} threads. Full counts on ER-16 and R-MAT 20: --workload cpu"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def timed_host_count(off, adj, threads: int, label: str):
+    """The oracle's hostCountTriangles (tests/cct.cpp:71-89) over the whole
+    graph, in 32 row slices of equal entry count so that a long single-thread
+    run prints its progress; returns (int64 total, seconds)."""
+    import oracle as O
+
+    nv = len(off) - 1
+    cuts = np.searchsorted(off, np.linspace(0, off[-1], 33).astype(np.int64), side="left")
+    cuts[0], cuts[-1] = 0, nv
+    total, spent, t_last = 0, 0.0, time.perf_counter()
+    for k in range(32):
+        v0, v1 = int(cuts[k]), int(cuts[k + 1])
+        if v1 <= v0:
+            continue
+        t0 = time.perf_counter()
+        total += O.host_count_rows(off, adj, v0, v1, threads)
+        spent += time.perf_counter() - t0
+        if time.perf_counter() - t_last > 20:
+            log(f"[bench] cpu {label}: rows < {v1} of {nv}, {spent:.0f} s")
+            t_last = time.perf_counter()
+    return total, spent
 
 
 def measured_traffic():
@@ -369,12 +407,42 @@ def bench_copy(T, args) -> dict:
             "cpu_baseline": None}
 
 
+def bench_cpu(T, args) -> dict:
+    """The reference CPU path (hostCountTriangles, tests/cct.cpp:71-89, as the
+    oracle restates it) run in full on ER-16 and R-MAT 20, single-threaded and
+    on every host core; each total is checked against 6 x the oracle's
+    degree-oriented count."""
+    import oracle as O
+
+    nth = O.nthreads()
+    runs = {}
+    for name, make in (("ER-16 (n=2^16, avg degree 16, seed 1)", lambda: O.er_graph(1 << 16, 16, 1)),
+                       ("R-MAT 20 (ef 16, seed 2)", lambda: O.rmat_graph(20, 16, 2))):
+        off, adj = make()
+        want = 6 * O.forward_count(off, adj)[0]
+        for th in (nth, 1):
+            total, dt = timed_host_count(off, adj, th, f"{name} {th}T")
+            if total != want:
+                raise RuntimeError(f"{name}: CPU total {total} != 6T = {want}")
+            runs[f"{name}, {th} threads"] = {"edges_per_s": len(adj) / dt, "seconds": dt,
+                                             "nnz": len(adj), "total_6T": total}
+            log(f"[bench] cpu {name} {th} threads: {dt:.2f} s, {len(adj) / dt:.3e} edges/s")
+    key = f"R-MAT 20 (ef 16, seed 2), {nth} threads"
+    return {"metric": "reference CPU TC edges/sec (hostCountTriangles, full count)",
+            "value": runs[key]["edges_per_s"], "unit": "edges/s",
+            "ms_per_step": runs[key]["seconds"] * 1e3, "higher_is_better": True,
+            "dtype": "int32",
+            "config": {"workload": "CPU reference path, full counts of ER-16 and R-MAT 20",
+                       "cpu_model": cpu_model(), "threads": nth, "runs": runs},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="static", choices=["static", "stream", "batched", "copy"])
+    ap.add_argument("--workload", default="static", choices=["static", "stream", "batched", "copy", "cpu"])
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--ef", type=int, default=32)
     ap.add_argument("--seed", type=int, default=4)
@@ -388,11 +456,13 @@ def main() -> int:
               "sharding of this path is a design, not a run", file=sys.stderr)
         return 2
 
-    import custingeralg_amd as T
-
-    log(f"[bench] device {T.device_name()}, workload {args.workload}")
+    if args.workload == "cpu":
+        T = None
+    else:
+        import custingeralg_amd as T
+        log(f"[bench] device {T.device_name()}, workload {args.workload}")
     fn = {"static": bench_static, "stream": bench_stream, "batched": bench_batched,
-          "copy": bench_copy}[args.workload]
+          "copy": bench_copy, "cpu": bench_cpu}[args.workload]
     rec = fn(T, args)
     line = {"metric": rec.pop("metric"), "value": rec.pop("value"), "unit": rec.pop("unit"),
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -49,7 +49,8 @@ EXPORTS = [
     "tc_gen_er_pairs", "tc_gen_rmat_multi", "tc_canon_csr_device", "tc_collect_timings", "tc_mlp_train_step",
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune", "tc_schedule_set_default",
     "tc_batched_graph_create", "tc_batched_graph_count", "tc_read_graph", "tc_host_free",
-    "tc_lb_edge_prefix", "tc_graph_get_view",
+    "tc_lb_edge_prefix", "tc_graph_get_view", "tc_batch_update_directed", "tc_graph_set_growth",
+    "tc_graph_compare", "tc_csr_count_device32",
 ]
 
 
@@ -63,8 +64,8 @@ class Schedule(C.Structure):
     _fields_ = [("method", C.c_int32), ("policy", C.c_int32), ("small_max", C.c_int32),
                 ("mid_max", C.c_int32), ("job_items", C.c_int32), ("ratio_x16", C.c_int32),
                 ("persistent", C.c_int32), ("flush_items", C.c_int32),
-                ("hub_degree", C.c_int32), ("mpath_min", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("hub_degree", C.c_int32), ("mpath_min", C.c_int32), ("capture", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):
@@ -72,7 +73,7 @@ class Stats(C.Structure):
                 ("ms_bin", C.c_double * 4), ("bytes_alg_bin", C.c_double * 4),
                 ("tasks_bin", C.c_int64 * 4), ("jobs_bin", C.c_int64 * 4),
                 ("strategy_tasks", C.c_int64 * 3), ("triangles", C.c_uint64),
-                ("method_used", C.c_int32), ("reserved0", C.c_int32),
+                ("method_used", C.c_int32), ("graph_replay", C.c_int32),
                 ("items_bin", C.c_double * 3), ("stream_bytes_bin", C.c_double * 3),
                 ("ms_long", C.c_double * 2), ("tasks_long", C.c_int64 * 2),
                 ("items_long", C.c_double * 2), ("stream_bytes_long", C.c_double * 2),
@@ -90,6 +91,7 @@ class Stats(C.Structure):
             "strategy_tasks": dict(zip(("merge", "bsearch", "hash"), self.strategy_tasks),
                                    mergepath=int(self.mergepath_tasks)),
             "triangles": int(self.triangles), "method_used": int(self.method_used),
+            "graph_replay": int(self.graph_replay),
         }
 
 
@@ -137,9 +139,13 @@ def lib():
         L.tc_schedule_default.argtypes = [C.POINTER(Schedule)]
         L.tc_static_count.argtypes = [vp, vp, C.POINTER(u64), C.POINTER(Schedule), C.POINTER(Stats)]
         L.tc_csr_count_device.argtypes = [i32, i64, vp, vp, vp, C.POINTER(u64), vp]
+        L.tc_csr_count_device32.argtypes = [i32, vp, vp, vp, C.POINTER(u64), vp]
         L.tc_batch_apply.argtypes = [vp, vp, vp, vp, i64, vp, C.POINTER(i64), C.POINTER(Schedule),
                                      C.POINTER(BatchStats)]
         L.tc_batch_update_only.argtypes = [vp, vp, vp, vp, i64, C.POINTER(BatchStats)]
+        L.tc_batch_update_directed.argtypes = [vp, vp, vp, vp, i64, C.POINTER(BatchStats)]
+        L.tc_graph_set_growth.argtypes = [vp, i32, i32]
+        L.tc_graph_compare.argtypes = [vp, vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i32)]
         L.tc_stream_delta_passes.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32]
         L.tc_batch_prepare.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.tc_batched_count.argtypes = [i32, vp, vp, vp, i64, i32, vp, vp, C.POINTER(Schedule),
@@ -281,6 +287,18 @@ class cuStinger:
         check(lib().tc_graph_to_csr(self.h, off.ctypes.data, adj.ctypes.data, TC_HOST))
         return off, adj[:nnz]
 
+    def set_growth(self, mul_x16: int = 0, add: int = 0) -> None:
+        """Update allocator policy: a row that outgrows its block moves to a
+        block of len * mul_x16 / 16 + add entries (0, 0: next power of two)."""
+        check(lib().tc_graph_set_growth(self.h, mul_x16, add))
+
+    def compare(self, other: "cuStinger") -> dict:
+        """compareCUS: rows whose lengths differ, rows whose entries differ,
+        the first differing vertex (-1: equal)."""
+        a, b, f = C.c_int64(0), C.c_int64(0), C.c_int32(0)
+        check(lib().tc_graph_compare(self.h, other.h, C.byref(a), C.byref(b), C.byref(f)))
+        return {"size_mismatch": a.value, "entry_mismatch": b.value, "first_vertex": f.value}
+
     def degrees(self) -> np.ndarray:
         d = DeviceArray(self.nv, np.int32)
         check(lib().tc_graph_degrees(self.h, d.ptr))
@@ -347,6 +365,25 @@ def batch_apply(g: cuStinger, u, v, is_delete=None, delta: DeviceArray | None = 
         return int(dg.value), st.as_dict()
     check(lib().tc_batch_update_only(g.h, du.ptr, dv.ptr, dd.ptr if dd else None, B, C.byref(st)))
     return 0, st.as_dict()
+
+
+def batch_update_directed(g: cuStinger, src, dst, is_delete=None) -> dict:
+    """edgeInsertions / edgeDeletions of directed entries (src[i] -> dst[i]),
+    as given; returns the batch stats."""
+    src = np.ascontiguousarray(src, np.int32)
+    dst = np.ascontiguousarray(dst, np.int32)
+    ds, dd = DeviceArray.from_host(src), DeviceArray.from_host(dst)
+    dl = DeviceArray.from_host(np.ascontiguousarray(is_delete, np.uint8)) \
+        if is_delete is not None else None
+    st = BatchStats()
+    try:
+        check(lib().tc_batch_update_directed(g.h, ds.ptr, dd.ptr, dl.ptr if dl else None, len(src),
+                                             C.byref(st)))
+    finally:
+        for a in (ds, dd, dl):
+            if a is not None:
+                a.free()
+    return st.as_dict()
 
 
 def batch_prepare(g: cuStinger, src, dst):

@@ -15,7 +15,9 @@ namespace tc {
 extern thread_local Error g_last;
 int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_del, int64_t B,
                 uint32_t *delta, int64_t *dglobal, const tc_schedule &sched, tc_batch_stats *st,
-                bool count);
+                bool count, bool directed);
+int graph_compare(Graph &a, Graph &b, int64_t *size_mismatch, int64_t *entry_mismatch,
+                  int32_t *first_vertex);
 int stream_delta_passes(Graph &g, int64_t nb, const int32_t *s_src, const int32_t *s_dst,
                         const int32_t *dup, const int64_t *boff, uint32_t *out, int32_t mask);
 int batch_prepare(Graph &g, int64_t nb, const int32_t *src, const int32_t *dst, int32_t *s_src,
@@ -271,6 +273,30 @@ int tc_csr_count_device(int32_t nv, int64_t nnz, const int64_t *d_off, const int
     })
 }
 
+int tc_csr_count_device32(int32_t nv, const int32_t *d_off, const int32_t *d_ind, uint32_t *out_dev,
+                          uint64_t *global_host, void *hip_stream)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(nv >= 0 && (nv == 0 || d_off), "tc_csr_count_device32: bad arguments");
+        int32_t nnz = 0;
+        if (nv > 0) {
+            TC_HIP(hipMemcpyAsync(&nnz, d_off + nv, sizeof(nnz), hipMemcpyDeviceToHost,
+                                  (hipStream_t)hip_stream));
+            TC_HIP(hipStreamSynchronize((hipStream_t)hip_stream));
+        }
+        TC_CHECK_ARG(nnz >= 0 && (nnz == 0 || d_ind), "tc_csr_count_device32: bad arguments");
+        tc::Graph *g = nullptr;
+        TC_TRY(tc::graph_create_csr(nv, nnz, reinterpret_cast<const int64_t *>(d_off), d_ind,
+                                    TC_DEVICE, TC_GRAPH_ASSUME_SYMMETRIC | TC_GRAPH_OFFSETS_I32, 0, &g));
+        g->stream = (hipStream_t)hip_stream;
+        tc_schedule s = tc::default_schedule();
+        s.method = TC_METHOD_DIRECT;
+        int st = tc::static_count(*g, out_dev, global_host, s, nullptr, true);
+        tc::graph_destroy(g);
+        return st;
+    })
+}
+
 int tc_batch_apply(tc_graph *g, const int32_t *u, const int32_t *v, const uint8_t *is_delete,
                    int64_t B, uint32_t *per_vertex_delta_dev, int64_t *delta_global_host,
                    const tc_schedule *sched, tc_batch_stats *stats)
@@ -281,7 +307,7 @@ int tc_batch_apply(tc_graph *g, const int32_t *u, const int32_t *v, const uint8_
         tc_schedule s;
         if (sched) s = *sched; else tc_schedule_default(&s);
         return tc::batch_apply(*g->g, u, v, is_delete, B, per_vertex_delta_dev, delta_global_host,
-                               s, stats, true);
+                               s, stats, true, false);
     })
 }
 
@@ -291,7 +317,39 @@ int tc_batch_update_only(tc_graph *g, const int32_t *u, const int32_t *v,
     TC_GUARD({
         TC_CHECK_ARG(g && B >= 0 && (B == 0 || (u && v)), "tc_batch_update_only: bad arguments");
         tc_schedule s = tc::default_schedule();
-        return tc::batch_apply(*g->g, u, v, is_delete, B, nullptr, nullptr, s, stats, false);
+        return tc::batch_apply(*g->g, u, v, is_delete, B, nullptr, nullptr, s, stats, false, false);
+    })
+}
+
+int tc_batch_update_directed(tc_graph *g, const int32_t *src, const int32_t *dst,
+                             const uint8_t *is_delete, int64_t B, tc_batch_stats *stats)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(g && B >= 0 && (B == 0 || (src && dst)),
+                     "tc_batch_update_directed: bad arguments");
+        tc_schedule s = tc::default_schedule();
+        return tc::batch_apply(*g->g, src, dst, is_delete, B, nullptr, nullptr, s, stats, false, true);
+    })
+}
+
+int tc_graph_set_growth(tc_graph *g, int32_t mul_x16, int32_t add)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(g && mul_x16 >= 0 && add >= 0 && (mul_x16 == 0 || mul_x16 >= 16 || add > 0),
+                     "tc_graph_set_growth: bad policy (mul_x16 >= 16, or a positive add)");
+        g->g->grow_mul_x16 = mul_x16;
+        g->g->grow_add = add;
+        return TC_OK;
+    })
+}
+
+int tc_graph_compare(tc_graph *a, tc_graph *b, int64_t *size_mismatch, int64_t *entry_mismatch,
+                     int32_t *first_vertex)
+{
+    TC_GUARD({
+        TC_CHECK_ARG(a && b && size_mismatch && entry_mismatch && first_vertex,
+                     "tc_graph_compare: bad arguments");
+        return tc::graph_compare(*a->g, *b->g, size_mismatch, entry_mismatch, first_vertex);
     })
 }
 

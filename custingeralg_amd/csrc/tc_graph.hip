@@ -56,6 +56,15 @@ int Graph::ensure_events()
     return TC_OK;
 }
 
+void Graph::drop_capture()
+{
+    if (cexec) (void)hipGraphExecDestroy(cexec);
+    if (cgraph) (void)hipGraphDestroy(cgraph);
+    cexec = nullptr;
+    cgraph = nullptr;
+    cexec_key = 0;
+}
+
 namespace {
 
 __global__ void k_len_from_off(const int64_t *__restrict__ off, int32_t nv, int32_t *__restrict__ len,
@@ -76,6 +85,14 @@ __global__ void k_len_from_off(const int64_t *__restrict__ off, int32_t nv, int3
             cap[v] = (int32_t)c;
         }
     }
+}
+
+// int32 offsets (the reference's length_t) widened to int64.
+__global__ void k_widen_off(const int32_t *__restrict__ in, int64_t n, int64_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
 }
 
 // Copy row v of a CSR into its block (blocks laid out by an exclusive scan of cap).
@@ -236,6 +253,13 @@ int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t 
         set_last_error(TC_ERR_INVALID, "tc_graph_create_csr: bad arguments");
         return TC_ERR_INVALID;
     }
+    const bool off32 = (flags & TC_GRAPH_OFFSETS_I32) != 0;
+    std::vector<int64_t> host_wide;
+    if (off32 && location == TC_HOST && nv > 0) {
+        const int32_t *o32 = reinterpret_cast<const int32_t *>(off);
+        host_wide.assign(o32, o32 + (size_t)nv + 1);
+        off = host_wide.data();
+    }
     if (location == TC_HOST && nv > 0) {
         if (off[0] != 0 || off[nv] != nnz) {
             set_last_error(TC_ERR_INVALID, "tc_graph_create_csr: off[0] must be 0 and off[nv] == nnz");
@@ -268,9 +292,16 @@ int graph_create_csr(int32_t nv, int64_t nnz, const int64_t *off, const int32_t 
     int64_t *doff = g->coff.as<int64_t>();
     hipStream_t s = nullptr;
     if (nv > 0) {
-        if (hipMemcpy(doff, off, (size_t)(nv + 1) * sizeof(int64_t),
-                      location == TC_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice) !=
-            hipSuccess) {
+        if (off32 && location == TC_DEVICE) {
+            hipLaunchKernelGGL(k_widen_off, dim3(grid_for((int64_t)nv + 1, 256, 65536)), dim3(256), 0, s,
+                               reinterpret_cast<const int32_t *>(off), (int64_t)nv + 1, doff);
+            if (hipGetLastError() != hipSuccess) {
+                set_last_error(TC_ERR_HIP, "widening of the offsets failed");
+                return fail(TC_ERR_HIP);
+            }
+        } else if (hipMemcpy(doff, off, (size_t)(nv + 1) * sizeof(int64_t),
+                             location == TC_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice) !=
+                   hipSuccess) {
             set_last_error(TC_ERR_HIP, "copy of offsets failed");
             return fail(TC_ERR_HIP);
         }
@@ -417,6 +448,70 @@ int graph_borrow_csr(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_
     return TC_OK;
 }
 
+namespace {
+
+// compareCUS / comparecus (src/streaming_triangle_counting/cct_upd.cu:442-484):
+// one wave per vertex compares the used lengths and then the entries of the
+// two stores' rows.  res[0] += rows whose lengths differ, res[1] += rows of
+// equal length whose entries differ, res[2] = max(nv - v) over differing
+// vertices v (0: none), i.e. the first differing vertex.
+__global__ void k_compare_rows(int32_t nv, const int64_t *__restrict__ boff_a,
+                               const int32_t *__restrict__ len_a, const int32_t *__restrict__ pool_a,
+                               const int64_t *__restrict__ boff_b,
+                               const int32_t *__restrict__ len_b, const int32_t *__restrict__ pool_b,
+                               unsigned long long *__restrict__ res)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    unsigned long long nsize = 0, nent = 0;
+    for (int64_t v = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < nv;
+         v += wstride) {
+        const int32_t la = len_a[v], lb = len_b[v];
+        bool diff = la != lb;
+        if (diff) {
+            nsize += lane == 0;
+        } else {
+            const int64_t ba = boff_a[v], bb = boff_b[v];
+            bool d = false;
+            for (int32_t i = lane; i < la; i += kWave) d |= pool_a[ba + i] != pool_b[bb + i];
+            diff = __ballot(d) != 0;
+            nent += (diff && lane == 0);
+        }
+        if (diff && lane == 0) atomicMax(&res[2], (unsigned long long)(nv - v));
+    }
+    const unsigned long long part[2] = {nsize, nent};
+    block_add_u64<2>(part, res);
+}
+
+}  // namespace
+
+int graph_compare(Graph &a, Graph &b, int64_t *size_mismatch, int64_t *entry_mismatch,
+                  int32_t *first_vertex)
+{
+    if (a.st.nv != b.st.nv) {
+        set_last_error(TC_ERR_INVALID, "tc_graph_compare: the stores have different vertex counts");
+        return TC_ERR_INVALID;
+    }
+    hipStream_t s = a.stream;
+    TC_HIP(hipStreamSynchronize(b.stream));
+    TC_TRY(a.accum.ensure(16 * sizeof(unsigned long long)));
+    unsigned long long *res = a.accum.as<unsigned long long>();
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, res, 3);
+    if (a.st.nv > 0)
+        hipLaunchKernelGGL(k_compare_rows, dim3(grid_for(a.st.nv, 4, 65536)), dim3(256), 0, s,
+                           a.st.nv, a.st.boff.as<int64_t>(), a.st.len.as<int32_t>(),
+                           a.st.pool.as<int32_t>(), b.st.boff.as<int64_t>(), b.st.len.as<int32_t>(),
+                           b.st.pool.as<int32_t>(), res);
+    TC_HIP(hipGetLastError());
+    unsigned long long h[3];
+    TC_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    *size_mismatch = (int64_t)h[0];
+    *entry_mismatch = (int64_t)h[1];
+    *first_vertex = h[2] == 0 ? -1 : (int32_t)((int64_t)a.st.nv - (int64_t)h[2]);
+    return TC_OK;
+}
+
 void graph_destroy(Graph *g)
 {
     if (!g) return;
@@ -436,6 +531,8 @@ void graph_destroy(Graph *g)
     for (DevBuf &b : g->lbuf) b.release();
     if (g->events_ready)
         for (auto &e : g->ev) (void)hipEventDestroy(e);
+    g->drop_capture();
+    if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
     delete g;
 }
 

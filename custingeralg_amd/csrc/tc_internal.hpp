@@ -147,6 +147,10 @@ struct Graph {
     // per-vertex marks, and batch scratch
     DevBuf flagpool, vmark;
     DevBuf sb[24];
+    // capacity of a row that outgrows its block: len * grow_mul_x16 / 16 +
+    // grow_add (the update allocator's policy); 0: the next power of two
+    int32_t grow_mul_x16 = 0;
+    int32_t grow_add = 0;
     DevBuf lbuf[3];  // load-balanced delta: items, their prefix, per-edge counts
     bool degree_stale = false;
     // batched counting: graph boundaries and per-graph output (device)
@@ -158,6 +162,36 @@ struct Graph {
     hipEvent_t ev[16] = {};
     bool events_ready = false;
     int ensure_events();
+    // store version: bumped by every update, so that plans and recorded
+    // counts of an older store are never reused
+    uint64_t version = 0;
+    // hipGraph replay of static counts (tc_schedule.capture).  A count
+    // records here the values the host reads back mid-sequence; a count with
+    // the same key then runs the sequence under stream capture with these
+    // values (capturing = true: no host read-backs, no events), and later
+    // ones launch the instantiated graph.
+    struct CountPlan {
+        bool valid = false;
+        uint64_t key = 0;
+        int method = 0;
+        unsigned long long hmax = 0;        // maximum degree (rank-sort bits)
+        int64_t nplus = 0, ntasks = 0;      // plus-array and task-array entries
+        std::vector<std::pair<int64_t, int64_t>> huge;  // N+ rows sorted in global memory: (rb, len)
+        int64_t jt[3] = {0, 0, 0};          // jobs per bin
+        int32_t *bad = nullptr;             // the orientation's symmetry flag (checked after a replay)
+    } plan;
+    bool capturing = false;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t cgraph = nullptr;
+    hipGraphExec_t cexec = nullptr;
+    uint64_t cexec_key = 0;
+    void drop_capture();
+    void touch()  // the store changed
+    {
+        ++version;
+        plan.valid = false;
+        drop_capture();
+    }
 };
 
 // Builds coff, crow and vinfo (and padj for the oriented layout) for the store;

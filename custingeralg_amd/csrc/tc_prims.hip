@@ -102,12 +102,16 @@ __global__ void k_tile_scan(const T *__restrict__ in, int64_t *__restrict__ out,
         *total = (tile_off ? tile_off[blockIdx.x] : 0) + tot;
 }
 
+__global__ void k_zero_total(int64_t *total) { *total = 0; }
+
 template <class T>
 int scan_impl(const T *in, int64_t *out, int64_t n, int64_t *total, DevBuf &tmp, hipStream_t s,
               size_t tmp_base)
 {
     if (n <= 0) {
-        if (total) TC_HIP(hipMemsetAsync(total, 0, sizeof(int64_t), s));
+        // a kernel, not a memset: counts may be recorded into a hipGraph
+        if (total) hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(1), 0, s, total);
+        TC_HIP(hipGetLastError());
         return TC_OK;
     }
     int64_t tiles = ceil_div(n, kScanTile);

@@ -59,17 +59,19 @@ __global__ void k_batch_check(const int32_t *__restrict__ u, const int32_t *__re
     }
 }
 
-// key = (min << (bv+1)) | (max << 1) | is_delete
+// key = (min << (bv+1)) | (max << 1) | is_delete; a directed batch keeps
+// (u, v) as given instead of (min, max).
 // Sort keys (pair, op) of the raw batch; ndel[0] (zeroed) receives the number
 // of deletion requests, in-batch repeats included.
 __global__ void k_batch_keys(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
-                             const uint8_t *__restrict__ del, int64_t B, int bv,
+                             const uint8_t *__restrict__ del, int64_t B, int bv, int directed,
                              uint64_t *__restrict__ keys, unsigned long long *__restrict__ ndel)
 {
     unsigned long long nd = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B;
          i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t a = (uint32_t)min(u[i], v[i]), b = (uint32_t)max(u[i], v[i]);
+        uint64_t a = (uint32_t)(directed ? u[i] : min(u[i], v[i]));
+        uint64_t b = (uint32_t)(directed ? v[i] : max(u[i], v[i]));
         const bool d = del && del[i];
         keys[i] = (a << (bv + 1)) | (b << 1) | (uint64_t)(d ? 1 : 0);
         nd += d;
@@ -113,21 +115,43 @@ __global__ void k_compact_pairs(const uint64_t *__restrict__ keys, int64_t n, in
 }
 
 // Locates (a,b) and (b,a) in the store; sel = 1 where the pair is present.
+// Directed: (a,b) only (pos_ba = -1).
 __global__ void k_find_pairs(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                              int64_t n, const int32_t *__restrict__ pool,
                              const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
                              int64_t *__restrict__ pos_ab, int64_t *__restrict__ pos_ba,
-                             int32_t *__restrict__ present)
+                             int32_t *__restrict__ present, int directed)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         int32_t a = pa[i], b = pb[i];
         int64_t x = find_pos(pool, boff[a], len[a], b);
-        int64_t y = find_pos(pool, boff[b], len[b], a);
+        int64_t y = directed ? -1 : find_pos(pool, boff[b], len[b], a);
         pos_ab[i] = x;
         pos_ba[i] = y;
-        present[i] = (x >= 0 && y >= 0) ? 1 : 0;
+        present[i] = (x >= 0 && (directed || y >= 0)) ? 1 : 0;
     }
+}
+
+// Directed updates of a symmetric store: counts the selected pairs (a,b)
+// whose reverse (b,a) is not in the state `want` (1 present, 0 absent) after
+// the update -- a nonzero count means the store may no longer be symmetric.
+__global__ void k_reverse_check(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+                                const int32_t *__restrict__ sel, int64_t n,
+                                const int32_t *__restrict__ pool, const int64_t *__restrict__ boff,
+                                const int32_t *__restrict__ len, int want,
+                                unsigned long long *__restrict__ bad)
+{
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!sel[i]) continue;
+        const int32_t a = pa[i], b = pb[i];
+        const bool has = find_pos(pool, boff[b], len[b], a) >= 0;
+        c += has != (want != 0);
+    }
+    const unsigned long long part[1] = {c};
+    block_add_u64<1>(part, bad);
 }
 
 __global__ void k_set_flags(const int64_t *__restrict__ pos_ab, const int64_t *__restrict__ pos_ba,
@@ -269,15 +293,21 @@ __global__ __launch_bounds__(kRowThreads) void k_compact_rows_block(
     }
 }
 
-// Directed keys (src << bv | dst) of the selected pairs, both directions.
+// Directed keys (src << bv | dst) of the selected pairs, both directions
+// (one, the pair as given, for a directed batch).
 __global__ void k_directed_keys(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                                 const int64_t *__restrict__ sidx, int64_t n,
-                                const int32_t *__restrict__ sel, int bv, uint64_t *__restrict__ keys)
+                                const int32_t *__restrict__ sel, int bv, int directed,
+                                uint64_t *__restrict__ keys)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (!sel[i]) continue;
         uint64_t a = (uint32_t)pa[i], b = (uint32_t)pb[i];
+        if (directed) {
+            keys[sidx[i]] = (a << bv) | b;
+            continue;
+        }
         int64_t p = 2 * sidx[i];
         keys[p] = (a << bv) | b;
         keys[p + 1] = (b << bv) | a;
@@ -311,21 +341,28 @@ __global__ void k_segments(const uint64_t *__restrict__ keys, const int32_t *__r
     }
 }
 
+// Block size of each affected row that outgrows its block (0: it fits):
+// nl * mul_x16 / 16 + add when the update allocator set a policy, else the
+// next power of two; never below nl, never above INT32_MAX.
 __global__ void k_need(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg,
                        int64_t n, const int32_t *__restrict__ len, const int32_t *__restrict__ cap,
-                       int32_t *__restrict__ need)
+                       int32_t mul_x16, int32_t add, int32_t *__restrict__ need)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
          s += (int64_t)gridDim.x * blockDim.x) {
         int64_t cnt = ((s + 1 < nseg) ? sbeg[s + 1] : n) - sbeg[s];
         int64_t nl = (int64_t)len[sv[s]] + cnt;
-        int32_t c = 0;
+        int64_t c = 0;
         if (nl > cap[sv[s]]) {
-            int64_t p = 1;
-            while (p < nl) p <<= 1;
-            c = (int32_t)p;
+            if (mul_x16 > 0) {
+                c = nl * mul_x16 / 16 + add;
+            } else {
+                c = 1;
+                while (c < nl) c <<= 1;
+            }
+            c = min<int64_t>(max<int64_t>(c, nl), INT32_MAX);
         }
-        need[s] = c;
+        need[s] = (int32_t)c;
     }
 }
 
@@ -856,9 +893,13 @@ int64_t read_i64(const int64_t *d, hipStream_t s)
 
 int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_del, int64_t B,
                 uint32_t *delta, int64_t *dglobal, const tc_schedule &sched, tc_batch_stats *stats,
-                bool count)
+                bool count, bool directed)
 {
     (void)sched;
+    if (directed && count) {
+        set_last_error(TC_ERR_INVALID, "a directed batch is applied without a triangle delta");
+        return TC_ERR_INVALID;
+    }
     const int32_t nv = g.st.nv;
     hipStream_t s = g.stream;
     tc_batch_stats bs = {};
@@ -871,12 +912,16 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         set_last_error(TC_ERR_UNSUPPORTED, "a borrowed (read-only) CSR view cannot be updated");
         return TC_ERR_UNSUPPORTED;
     }
-    if (g.symmetric < 0 || g.self_loops < 0) {
+    g.touch();
+    // undirected updates keep a symmetric store symmetric; a directed batch
+    // (edgeInsertions / edgeDeletions of its entries as given) takes any store
+    if (!directed && (g.symmetric < 0 || g.self_loops < 0)) {
         int64_t bad = 0;
         TC_TRY(validate_graph(g, true, &bad));
     }
-    if (g.symmetric != 1 || g.self_loops != 0) {
-        set_last_error(TC_ERR_INVALID, "batch updates need a symmetric store without self-loops");
+    if (!directed && (g.symmetric != 1 || g.self_loops != 0)) {
+        set_last_error(TC_ERR_INVALID, "undirected batch updates need a symmetric store without "
+                                       "self-loops (directed stores take directed batches)");
         return TC_ERR_INVALID;
     }
     TC_TRY(g.ensure_events());
@@ -888,12 +933,16 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     unsigned long long hb[2];
     TC_HIP(hipMemcpyAsync(hb, acc, sizeof(hb), hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
-    if (hb[0] || hb[1]) {
+    if (hb[0] || (!directed && hb[1])) {
         set_last_error(TC_ERR_INVALID, "batch has " + std::to_string(hb[0]) +
                                            " out-of-range ids and " + std::to_string(hb[1]) +
                                            " self-loops; nothing applied");
         return TC_ERR_INVALID;
     }
+    // a directed batch keeps a symmetric store's flag only if every touched
+    // pair's reverse ends in the same state (acc[6] counts those that do not)
+    const bool track_sym = directed && g.symmetric == 1;
+    if (directed && hb[1]) g.self_loops = -1;
     TC_TRY(ensure_flags(g));
     TC_HIP(hipEventRecord(g.ev[8], s));
 
@@ -905,7 +954,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     TC_NEED(keys);
     // acc[5]: deletion requests of the raw batch
     hipLaunchKernelGGL(k_batch_keys, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, u, v, is_del,
-                       B, bv, keys, acc + 5);
+                       B, bv, directed ? 1 : 0, keys, acc + 5);
     TC_TRY(radix_sort_u64(keys, nullptr, B, bits, g.sb[1], g.sb[2], g.tmp, s));
     int32_t *fi = buf<int32_t>(g, 3, B), *fd = buf<int32_t>(g, 4, B);
     int64_t *pi = buf<int64_t>(g, 5, B + 1), *pd = buf<int64_t>(g, 6, B + 1);
@@ -937,7 +986,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_NEED(pab && pba && pres && queue);
         hipLaunchKernelGGL(k_find_pairs, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, da, db, nD,
                            g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                           pab, pba, pres);
+                           pab, pba, pres, directed ? 1 : 0);
         hipLaunchKernelGGL(k_set_flags, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, pab, pba,
                            pres, nD, (uint8_t)1, g.flagpool.as<uint8_t>());
         if (count) {
@@ -973,7 +1022,11 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_NEED(cnt);
         TC_TRY(scan_counts_i32(pres, cnt, nD, g.tmp, s));
         bs.deleted = read_i64(cnt + nD, s);
-        g.st.nnz -= 2 * bs.deleted;
+        g.st.nnz -= (directed ? 1 : 2) * bs.deleted;
+        if (track_sym)  // every deleted (a,b) needs (b,a) gone as well
+            hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(nD, 256, 4096)), dim3(256), 0, s, da, db,
+                               pres, nD, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                               g.st.len.as<int32_t>(), 0, acc + 6);
     }
 
     // ---------------------------------------------------------- insertions
@@ -984,19 +1037,19 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_NEED(pab && pba && pres && absent && sidx);
         hipLaunchKernelGGL(k_find_pairs, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, ia, ib, nI,
                            g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                           pab, pba, pres);
+                           pab, pba, pres, directed ? 1 : 0);
         hipLaunchKernelGGL(k_not, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, pres, nI, absent);
         TC_TRY(scan_counts_i32(absent, sidx, nI, g.tmp, s));
         const int64_t nNew = read_i64(sidx + nI, s);
         bs.inserted = nNew;
         if (nNew > 0) {
-            const int64_t nd = 2 * nNew;
+            const int64_t nd = (directed ? 1 : 2) * nNew;
             uint64_t *dk = buf<uint64_t>(g, 18, nd);
             int32_t *head = buf<int32_t>(g, 19, nd);
             int64_t *hpos = buf<int64_t>(g, 20, nd + 1);
             TC_NEED(dk && head && hpos);
             hipLaunchKernelGGL(k_directed_keys, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, ia, ib,
-                               sidx, nI, absent, bv, dk);
+                               sidx, nI, absent, bv, directed ? 1 : 0, dk);
             TC_TRY(radix_sort_u64(dk, nullptr, nd, 2 * bv, g.sb[1], g.sb[2], g.tmp, s));
             hipLaunchKernelGGL(k_segment_heads, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, dk, nd,
                                bv, head);
@@ -1011,7 +1064,8 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
                                hpos, nd, bv, nseg, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), sv,
                                sbeg, need);
             hipLaunchKernelGGL(k_need, dim3(grid_for(nseg, 256, 65536)), dim3(256), 0, s, sv, sbeg,
-                               nseg, nd, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), need);
+                               nseg, nd, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
+                               g.grow_mul_x16, g.grow_add, need);
             TC_TRY(scan_counts_i32(need, nscan, nseg, g.tmp, s));
             const int64_t grow = read_i64(nscan + nseg, s);
             TC_TRY(ensure_pool(g, grow));
@@ -1045,6 +1099,10 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
             }
             hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, newpos, nd,
                                g.flagpool.as<uint8_t>());
+            if (track_sym)  // every inserted (a,b) needs (b,a) present
+                hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(nI, 256, 4096)), dim3(256), 0, s, ia,
+                                   ib, absent, nI, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                                   g.st.len.as<int32_t>(), 1, acc + 6);
             TC_HIP(hipGetLastError());
             TC_HIP(hipStreamSynchronize(s));
             if (count) {
@@ -1059,9 +1117,10 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     bs.ms_delta = ms_delta;
     bs.ms_update = ms - ms_delta;
     // no-ops: requests (in-batch repeats included) that changed nothing
-    unsigned long long h[4] = {0, 0, 0, 0};
+    unsigned long long h[5] = {0, 0, 0, 0, 0};
     TC_HIP(hipMemcpy(h, acc + 2, sizeof(h), hipMemcpyDeviceToHost));
     const int64_t del_requests = (int64_t)h[3];
+    if (track_sym && h[4]) g.symmetric = -1;  // revalidated when next needed
     bs.missing_deletes = del_requests - bs.deleted;
     bs.dup_inserts = (B - del_requests) - bs.inserted;
     if (count && dglobal) *dglobal = (int64_t)h[0];

@@ -384,4 +384,16 @@ int allEinA_TraverseEdges(tc_graph *g, void *metadata, const int32_t *src, const
     return for_batch_edges(g, src, dst, n, ops_detail::EdgeFn<F>{metadata});
 }
 
+// The reference's batch form (operators.cuh:333-344): the size from
+// bu.getHostBUD()->getBatchSize(), the device entries from
+// bu.getDeviceBUD()->getSrc()/getDst() (a BatchUpdate of custinger_tc.hpp).
+template <cusSubKernelEdge F, class Batch>
+int allEinA_TraverseEdges(tc_graph *g, void *metadata, Batch &bu)
+{
+    const int64_t n = *(bu.getHostBUD()->getBatchSize());
+    if (n == 0) return TC_OK;
+    const auto *d = bu.getDeviceBUD();
+    return for_batch_edges(g, d->getSrc(), d->getDst(), n, ops_detail::EdgeFn<F>{metadata});
+}
+
 }  // namespace cuStingerAlgs

@@ -64,6 +64,7 @@ typedef struct tc_graph tc_graph;
 /* Create flags. */
 #define TC_GRAPH_ASSUME_SYMMETRIC 0x1u /* skip the symmetry check (caller vouches) */
 #define TC_GRAPH_NO_VALIDATE      0x2u /* skip sortedness/range validation        */
+#define TC_GRAPH_OFFSETS_I32      0x4u /* off is int32[nv+1] (the reference's length_t) */
 
 /* cuStinger::initializeCuStinger with eInitStateCSR (tests/cct.cpp:261-277):
  * build the dynamic per-vertex block store from a CSR (off[nv+1] int64,
@@ -87,6 +88,21 @@ typedef struct {
     int64_t reallocations;     /* block moves since creation ("allocs", :392,398)         */
 } tc_graph_info;
 int tc_graph_get_info(tc_graph *g, tc_graph_info *info);
+
+/* Block growth policy, the update allocator of cuStinger(initAlloc,
+ * updateAlloc) (tests/cct.cpp:261 passes defaultUpdateAllocater, 2x): a row
+ * that outgrows its block moves to a block of len * mul_x16 / 16 + add
+ * entries (at least len).  mul_x16 = add = 0 restores the default, the next
+ * power of two. */
+int tc_graph_set_growth(tc_graph *g, int32_t mul_x16, int32_t add);
+
+/* compareCUS (src/streaming_triangle_counting/cct_upd.cu:442-484): compares
+ * two stores of the same vertex count row by row on the device.
+ * size_mismatch receives the rows whose used lengths differ, entry_mismatch
+ * the rows of equal length whose entries differ, first_vertex the first
+ * differing vertex (-1 if the stores are equal). */
+int tc_graph_compare(tc_graph *a, tc_graph *b, int64_t *size_mismatch, int64_t *entry_mismatch,
+                     int32_t *first_vertex);
 
 /* Snapshot of the store as a compact CSR (location TC_HOST or TC_DEVICE);
  * off has nv+1 entries, adj room for info.nnz entries. */
@@ -127,7 +143,12 @@ typedef struct {
     int32_t mpath_min;     /* fixed heuristic: balanced pairs (long <= 2 * short)     */
                            /*   whose shorter list has at least this many entries    */
                            /*   run as a merge path (0, the default: never)          */
-    int32_t reserved[6];
+    int32_t capture;       /* 1: repeated counts of an unchanged graph with the same  */
+                           /*   schedule and output run as one hipGraph launch: the  */
+                           /*   second such count records the whole sequence, later  */
+                           /*   ones replay it (no host read-backs in between; the   */
+                           /*   stats then give ms_total only, graph_replay = 1)     */
+    int32_t reserved[5];
 } tc_schedule;
 
 /* Fill *s with the defaults (autotuned table if one was loaded). */
@@ -144,7 +165,7 @@ typedef struct {
     int64_t strategy_tasks[3]; /* tasks run as merge / bsearch / hash         */
     uint64_t triangles;     /* T for simple graphs (global/6), else 0         */
     int32_t method_used;
-    int32_t reserved0;
+    int32_t graph_replay;   /* 1: this count was a hipGraph replay (capture)  */
     double items_bin[3];    /* task-list entries the tasks of each bin stream */
     double stream_bytes_bin[3]; /* bytes each bin must read: 4 per streamed
                                    entry + 24 per task (task entry and vertex
@@ -174,6 +195,11 @@ int tc_static_count(tc_graph *g, uint32_t *per_vertex_dev, uint64_t *global_host
  * skipped, sum = 3T).  d_off int64[nv+1], d_ind int32[nnz], out uint32[nv]. */
 int tc_csr_count_device(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_t *d_ind,
                         uint32_t *out_dev, uint64_t *global_host, void *hip_stream);
+/* The same with the reference's int32 offsets (const length_t *d_off,
+ * tests/cct_csr.cu:279-285): d_off int32[nv+1] on the device, widened there;
+ * nnz = d_off[nv]. */
+int tc_csr_count_device32(int32_t nv, const int32_t *d_off, const int32_t *d_ind, uint32_t *out_dev,
+                          uint64_t *global_host, void *hip_stream);
 
 /* ------------------------------------------------------ batch update -- */
 typedef struct {
@@ -202,9 +228,23 @@ int tc_batch_apply(tc_graph *g, const int32_t *u, const int32_t *v, const uint8_
                    int64_t B, uint32_t *per_vertex_delta_dev, int64_t *delta_global_host,
                    const tc_schedule *sched, tc_batch_stats *stats);
 
-/* Store-only update (no counting): cuStinger edgeInsertions / edgeDeletions. */
+/* Store-only update (no counting): cuStinger edgeInsertions / edgeDeletions
+ * of undirected pairs (both directions of each pair) on a symmetric store. */
 int tc_batch_update_only(tc_graph *g, const int32_t *u, const int32_t *v,
                          const uint8_t *is_delete, int64_t B, tc_batch_stats *stats);
+
+/* cuStinger edgeInsertions / edgeDeletions of DIRECTED entries, as
+ * tests/katz-tester.cu:211-252 applies them to a directed store: each entry
+ * (src[i], dst[i]) (device int32[B]) is inserted into, or with is_delete[i]
+ * (device uint8[B], or NULL: all insertions) removed from, the row of src[i]
+ * only.  Any store (symmetric or not, self-loops allowed); in-batch repeats,
+ * present insertions and absent deletions are no-ops counted in the stats.
+ * Deletions are applied before insertions.  A symmetric store stays flagged
+ * symmetric when every touched entry's reverse ends in the same state;
+ * otherwise symmetry is re-checked when next needed (tc_batch_apply refuses
+ * an asymmetric store; tc_static_count counts it with the direct method). */
+int tc_batch_update_directed(tc_graph *g, const int32_t *src, const int32_t *dst,
+                             const uint8_t *is_delete, int64_t B, tc_batch_stats *stats);
 
 /* BatchUpdate::sortDeviceBUD (tests/cct_upd.cpp:395) with the duplicate
  * flags the delta kernels consume (cct_upd.cu:252,308,364): the nb directed

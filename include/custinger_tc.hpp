@@ -25,7 +25,9 @@
 #ifndef CUSTINGER_TC_HPP
 #define CUSTINGER_TC_HPP
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -126,6 +128,15 @@ public:
         if (slack < 0) slack = 0;
         tcCheck(tc_graph_create_csr(nv, nnz, off.data(), nnz ? cfg.csrAdj : nullptr, TC_HOST, 0, slack,
                                     &g_));
+        // growth of rows that outgrow their blocks: the update allocator,
+        // read as the line a*x + b through two sample sizes
+        const double x1 = 1024.0, x2 = 1048576.0;
+        const double f1 = (double)updateAlloc_((length_t)x1), f2 = (double)updateAlloc_((length_t)x2);
+        const double a = (f2 - f1) / (x2 - x1);
+        int32_t mul_x16 = (int32_t)(a * 16.0 + 0.5);
+        int32_t add = (int32_t)std::max(0.0, f1 - a * x1 + 0.5);
+        if (mul_x16 < 16 && add == 0) mul_x16 = 16;  // never below the used length
+        tcCheck(tc_graph_set_growth(g_, mul_x16, add));
     }
     void freecuStinger()
     {
@@ -137,9 +148,11 @@ public:
     int64_t getNumberEdgesUsed() const { return info().nnz; }
     int64_t getNumberEdgesAllocated() const { return info().allocated; }
 
-    // Applies a prepared batch as insertions; allocs = blocks moved to grow.
+    // Inserts the batch's directed entries as given (allocs = blocks moved to
+    // grow); the per-vertex duplicate counts of its BatchUpdateData are filled.
     void edgeInsertionsSorted(BatchUpdate &bu, length_t &allocs);
     void edgeInsertions(BatchUpdate &bu, length_t &allocs) { edgeInsertionsSorted(bu, allocs); }
+    // Removes the batch's directed entries (tests/katz-tester.cu:244).
     void edgeDeletions(BatchUpdate &bu);
 
     tc_graph_info info() const
@@ -157,8 +170,11 @@ private:
     cuStingerAllocator initAlloc_, updateAlloc_;
 };
 
-// Host-side batch of directed entries (the reference fills both directions,
-// tests/cct_upd.cpp:337-341).
+// Host-side batch of directed entries (the reference fills both directions
+// for an undirected graph, tests/cct_upd.cpp:337-341; katz-tester's directed
+// batches one, tests/katz-tester.cu:211-222).  getvNumDuplicates()[v] counts
+// the entries with source v that edgeInsertions found already present or
+// repeated in the batch (tests/cct_upd.cpp:335 zeroes it before the update).
 class BatchUpdateData {
 public:
     BatchUpdateData(length_t batchSize, bool isHost = true, length_t numVertices = 0)
@@ -181,10 +197,22 @@ private:
     length_t size_ = (length_t)src_.size();
 };
 
+// The batch's device arrays in the shape of the reference's device-side
+// BatchUpdateData (getDeviceBUD()->..., include/operators.cuh:334-344): the
+// getters return device pointers; devicePtr() returns this host view.
+struct BatchUpdateDeviceData {
+    vertexId_t *src = nullptr, *dst = nullptr;
+    length_t size = 0;
+    vertexId_t *getSrc() const { return src; }
+    vertexId_t *getDst() const { return dst; }
+    const length_t *getBatchSize() const { return &size; }
+    const BatchUpdateDeviceData *devicePtr() const { return this; }
+};
+
 // Device copy of a batch plus its sorted form and duplicate flags.
 class BatchUpdate {
 public:
-    explicit BatchUpdate(BatchUpdateData &bud) : nb_(bud.size())
+    explicit BatchUpdate(BatchUpdateData &bud) : nb_(bud.size()), bud_(&bud)
     {
         src_ = alloc<int32_t>(nb_);
         dst_ = alloc<int32_t>(nb_);
@@ -213,6 +241,14 @@ public:
     // store) and batch offsets against the store's current contents.
     void prepare(cuStinger &custing);
     length_t getBatchSize() const { return nb_; }
+    BatchUpdateData *getHostBUD() { return bud_; }
+    BatchUpdateDeviceData *getDeviceBUD()
+    {
+        dview_.src = src_;
+        dview_.dst = dst_;
+        dview_.size = nb_;
+        return &dview_;
+    }
     bool prepared() const { return prepared_; }
     bool sortRequested() const { return sort_requested_; }
     const int32_t *deviceSrc() const { return src_; }
@@ -230,6 +266,8 @@ private:
         return static_cast<T *>(p);
     }
     length_t nb_;
+    BatchUpdateData *bud_;
+    BatchUpdateDeviceData dview_;
     int32_t *src_ = nullptr, *dst_ = nullptr, *ssrc_ = nullptr, *sdst_ = nullptr, *dup_ = nullptr;
     int64_t *boff_ = nullptr;
     bool prepared_ = false;
@@ -253,10 +291,22 @@ inline void BatchUpdate::prepare(cuStinger &custing)
 inline void cuStinger::edgeInsertionsSorted(BatchUpdate &bu, length_t &allocs)
 {
     bu.prepare(*this);  // flags against the pre-insert store, as the delta passes need
-    // the directed batch holds both directions; the store inserts each pair once
+    const int64_t nb = bu.getBatchSize();
+    // per-vertex duplicates: entries the insertion leaves alone, by source
+    BatchUpdateData *bud = bu.getHostBUD();
+    const length_t nvd = bud ? *bud->getNumVertices() : 0;
+    if (nb > 0 && nvd > 0) {
+        std::vector<int32_t> ssrc((size_t)nb), dup((size_t)nb);
+        copyArrayDeviceToHost(bu.sortedSrc(), ssrc.data(), nb, 4);
+        copyArrayDeviceToHost(bu.duplicateFlags(), dup.data(), nb, 4);
+        length_t *nd = bud->getvNumDuplicates();
+        for (int64_t i = 0; i < nb; ++i)
+            if (dup[(size_t)i] && ssrc[(size_t)i] < nvd) ++nd[ssrc[(size_t)i]];
+    }
+    // the entries as given: both directions of an undirected batch, one of a
+    // directed one
     tc_batch_stats st;
-    tcCheck(tc_batch_update_only(g_, bu.deviceSrc(), bu.deviceDst(), nullptr, bu.getBatchSize(),
-                                 &st));
+    tcCheck(tc_batch_update_directed(g_, bu.deviceSrc(), bu.deviceDst(), nullptr, nb, &st));
     allocs = (length_t)st.reallocations;
 }
 
@@ -269,9 +319,24 @@ inline void cuStinger::edgeDeletions(BatchUpdate &bu)
     int st = tc_memcpy(d, del.data(), del.size(), 0);
     tc_batch_stats bs;
     if (st == TC_OK)
-        st = tc_batch_update_only(g_, bu.deviceSrc(), bu.deviceDst(), (const uint8_t *)d, nb, &bs);
+        st = tc_batch_update_directed(g_, bu.deviceSrc(), bu.deviceDst(), (const uint8_t *)d, nb, &bs);
     tc_device_free(d);
     tcCheck(st);
+}
+
+// compareCUS (src/streaming_triangle_counting/cct_upd.cu:476-484): compares the
+// two stores on the device and, like the reference's kernel, prints the
+// mismatches (first differing vertex and counts); returns the number of
+// differing rows.
+inline int64_t compareCUS(cuStinger *cus1, cuStinger *cus2)
+{
+    int64_t sz = 0, ent = 0;
+    int32_t first = -1;
+    tcCheck(tc_graph_compare(cus1->handle(), cus2->handle(), &sz, &ent, &first));
+    if (sz || ent)
+        std::printf("stores differ: %lld rows of different length, %lld rows with different "
+                    "entries, first at vertex %d\n", (long long)sz, (long long)ent, first);
+    return sz + ent;
 }
 
 // ----------------------------------------------------- compat functions --
@@ -317,17 +382,9 @@ inline void callDeviceAllTrianglesCSR(const vertexId_t nv, const length_t *d_off
                                       const int shifter, const int thread_blocks, const int blockdim)
 {
     (void)threads_per_block, (void)number_blocks, (void)shifter, (void)thread_blocks, (void)blockdim;
-    std::vector<int32_t> h((size_t)nv + 1);
-    copyArrayDeviceToHost(d_off, h.data(), nv + 1, 4);
-    std::vector<int64_t> h64(h.begin(), h.end());
-    void *d64 = nullptr;
-    tcCheck(tc_device_alloc(&d64, h64.size() * 8));
-    int st = tc_memcpy(d64, h64.data(), h64.size() * 8, 0);
-    if (st == TC_OK)
-        st = tc_csr_count_device(nv, h64[(size_t)nv], (const int64_t *)d64, d_ind,
-                                 reinterpret_cast<uint32_t *>(outPutTriangles), nullptr, nullptr);
-    tc_device_free(d64);
-    tcCheck(st);
+    // int32 offsets go to the library as they are (widened on the device)
+    tcCheck(tc_csr_count_device32(nv, d_off, d_ind, reinterpret_cast<uint32_t *>(outPutTriangles),
+                                  nullptr, nullptr));
 }
 
 // readGraphDIMACS / readGraphSNAP / readGraphMatrixMarket with the drivers'

@@ -196,6 +196,7 @@ static void insertion_test()
     std::vector<triangle_t> zero((size_t)nv + 1, 0);
     copyArrayHostToDevice(zero.data(), d_new, nv + 1, sizeof(triangle_t));
     length_t total_allocs = 0;
+    std::set<std::pair<int32_t, int32_t>> store = reduced;  // the store's entries before each batch
     for (int b = 0; b < numBatches; ++b) {
         const auto &B = batches[(size_t)b];
         const length_t nt = (length_t)B.size();
@@ -206,10 +207,25 @@ static void insertion_test()
             bud.getSrc()[nt + i] = B[(size_t)i].second;  // reversed copies (:337-341)
             bud.getDst()[nt + i] = B[(size_t)i].first;
         }
+        // expected getvNumDuplicates: per source, the entries the insertion
+        // leaves alone (repeats after the first, entries already present)
+        std::vector<length_t> want_dup((size_t)nv, 0);
+        {
+            std::set<std::pair<int32_t, int32_t>> seen;
+            for (length_t i = 0; i < 2 * nt; ++i) {
+                const std::pair<int32_t, int32_t> e{bud.getSrc()[i], bud.getDst()[i]};
+                if (store.count(e) || !seen.insert(e).second) want_dup[(size_t)e.first]++;
+            }
+            store.insert(seen.begin(), seen.end());
+        }
+        for (int32_t i = 0; i < nv; ++i) bud.getvNumDuplicates()[i] = 0;  // tests/cct_upd.cpp:335
         BatchUpdate bu1(bud);
         length_t allocs = 0;
         bu1.sortDeviceBUD(128);
         custingTest.edgeInsertionsSorted(bu1, allocs);
+        bool dup_ok = true;
+        for (int32_t i = 0; i < nv; ++i) dup_ok = dup_ok && bud.getvNumDuplicates()[i] == want_dup[(size_t)i];
+        CHECK(dup_ok, "getvNumDuplicates after batch %d", b);
         callDeviceNewTriangles(custingTest, bu1, d_new, 1, 128, 0, 16000, 128, nullptr, nullptr);
         total_allocs += allocs;
     }
@@ -254,6 +270,85 @@ static void insertion_test()
     CHECK(hp == pv_red, "streaming per-vertex after deletions");
     st.Release();
     custingTest.freecuStinger();
+}
+
+// tests/katz-tester.cu:211-252 on a directed graph: a directed batch inserted
+// with edgeInsertions and removed with edgeDeletions; after each step the
+// store equals the expected one (compareCUS against a store built from the
+// expected CSR), and the static count of the directed store equals the
+// reference formula's.
+static void directed_test()
+{
+    const int32_t nv = 2000;
+    std::set<std::pair<int32_t, int32_t>> edges;
+    uint64_t ctr = 0;
+    auto rnd = [&](uint64_t n) { return (int32_t)(tco_rng(200, ctr++) % n); };
+    while (edges.size() < 20000) {
+        const int32_t a = rnd(nv), b = rnd(nv);
+        if (a != b) edges.insert({a, b});  // one direction only: a directed graph
+    }
+    const Csr g0 = from_set(nv, edges);
+    std::vector<length_t> off32;
+    cuStinger custing(defaultInitAllocater, defaultUpdateAllocater);
+    init_custinger(custing, g0, off32);
+    CHECK(custing.info().symmetric == 0, "directed store flagged asymmetric");
+    // generateEdgeUpdates shape: uniform directed entries, repeats allowed
+    const length_t nb = 3000;
+    BatchUpdateData bud(nb, true);
+    std::set<std::pair<int32_t, int32_t>> batch;
+    for (length_t i = 0; i < nb; ++i) {
+        int32_t a = rnd(nv), b = rnd(nv);
+        if (a == b) b = (b + 1) % nv;
+        if (i % 10 == 0 && i > 0) {  // every tenth entry repeats an earlier one
+            a = bud.getSrc()[i / 2];
+            b = bud.getDst()[i / 2];
+        }
+        bud.getSrc()[i] = a;
+        bud.getDst()[i] = b;
+        batch.insert({a, b});
+    }
+    BatchUpdate bu(bud);
+    length_t allocs = 0;
+    custing.edgeInsertions(bu, allocs);
+    std::set<std::pair<int32_t, int32_t>> after = edges;
+    after.insert(batch.begin(), batch.end());
+    const Csr g1 = from_set(nv, after);
+    {
+        cuStinger want;
+        std::vector<length_t> o;
+        init_custinger(want, g1, o);
+        CHECK(compareCUS(&custing, &want) == 0, "directed insertions");
+        CHECK(compareCUS(&custing, &custing) == 0, "a store equals itself");
+    }
+    CHECK(custing.getNumberEdgesUsed() == (int64_t)after.size(), "entries after directed insertions");
+    // counting a directed store: the reference formula over the rows as they are
+    triangle_t *d = (triangle_t *)allocDeviceArray(nv, sizeof(triangle_t));
+    callDeviceAllTriangles(custing, d, 1, 128, 0, 16000, 128);
+    std::vector<triangle_t> h((size_t)nv);
+    copyArrayDeviceToHost(d, h.data(), nv, sizeof(triangle_t));
+    const std::vector<uint32_t> want_pv = oracle_pv(g1);
+    bool same = true;
+    for (int32_t i = 0; i < nv; ++i) same = same && (uint32_t)h[(size_t)i] == want_pv[(size_t)i];
+    CHECK(same, "count of the directed store");
+    freeDeviceArray(d);
+    custing.edgeDeletions(bu);
+    std::set<std::pair<int32_t, int32_t>> left = edges;
+    for (auto &e : batch) left.erase(e);
+    const Csr g2 = from_set(nv, left);
+    {
+        cuStinger want;
+        std::vector<length_t> o;
+        init_custinger(want, g2, o);
+        CHECK(compareCUS(&custing, &want) == 0, "directed deletions");
+        cuStinger other;
+        init_custinger(other, g0, o);
+        const bool differs = !batch.empty() && left.size() != edges.size();
+        CHECK(!differs || compareCUS(&custing, &other) > 0, "compareCUS finds a difference");
+    }
+    std::printf("directed test: %zu entries, batch %d (%zu distinct), %zu after insert, %zu after "
+                "delete, allocs %d\n",
+                edges.size(), nb, batch.size(), after.size(), left.size(), allocs);
+    custing.freecuStinger();
 }
 
 // readGraphDIMACS -> initializeCuStinger -> callDeviceAllTriangles, as the
@@ -307,6 +402,7 @@ int main()
     try {
         static_sweep();
         insertion_test();
+        directed_test();
         file_test();
     } catch (const cuStingerError &e) {
         std::printf("FAIL exception: %s\n", e.what());
