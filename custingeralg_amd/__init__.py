@@ -50,7 +50,7 @@ EXPORTS = [
     "tc_mlp_forward", "tc_model_load", "tc_model_table", "tc_autotune", "tc_schedule_set_default",
     "tc_batched_graph_create", "tc_batched_graph_count", "tc_read_graph", "tc_host_free",
     "tc_lb_edge_prefix", "tc_graph_get_view", "tc_batch_update_directed", "tc_graph_set_growth",
-    "tc_graph_compare", "tc_csr_count_device32",
+    "tc_graph_compare", "tc_csr_count_device32", "tc_csr_count_device_noalloc",
 ]
 
 
@@ -140,6 +140,7 @@ def lib():
         L.tc_static_count.argtypes = [vp, vp, C.POINTER(u64), C.POINTER(Schedule), C.POINTER(Stats)]
         L.tc_csr_count_device.argtypes = [i32, i64, vp, vp, vp, C.POINTER(u64), vp]
         L.tc_csr_count_device32.argtypes = [i32, vp, vp, vp, C.POINTER(u64), vp]
+        L.tc_csr_count_device_noalloc.argtypes = [i32, i64, vp, vp, vp, C.POINTER(u64), vp]
         L.tc_batch_apply.argtypes = [vp, vp, vp, vp, i64, vp, C.POINTER(i64), C.POINTER(Schedule),
                                      C.POINTER(BatchStats)]
         L.tc_batch_update_only.argtypes = [vp, vp, vp, vp, i64, C.POINTER(BatchStats)]

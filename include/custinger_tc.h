@@ -195,6 +195,18 @@ int tc_static_count(tc_graph *g, uint32_t *per_vertex_dev, uint64_t *global_host
  * skipped, sum = 3T).  d_off int64[nv+1], d_ind int32[nnz], out uint32[nv]. */
 int tc_csr_count_device(int32_t nv, int64_t nnz, const int64_t *d_off, const int32_t *d_ind,
                         uint32_t *out_dev, uint64_t *global_host, void *hip_stream);
+/* No counterpart in the reference: tc_csr_count_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, output and
+ * errors; the caller's CSR is read in place, and the only scratch is 256
+ * bytes of counters in a static device array per GPU (validation flags, a
+ * work cursor, the global sum), so no graph is too large for it.  Calls no
+ * allocating routine.  Calls that overlap on one GPU share that scratch and
+ * race; serialise them (one stream, or external locking). */
+int tc_csr_count_device_noalloc(int32_t nv, int64_t nnz, const int64_t *d_off,
+                                const int32_t *d_ind, uint32_t *out_dev, uint64_t *global_host,
+                                void *hip_stream);
+
 /* The same with the reference's int32 offsets (const length_t *d_off,
  * tests/cct_csr.cu:279-285): d_off int32[nv+1] on the device, widened there;
  * nnz = d_off[nv]. */
