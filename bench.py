@@ -192,9 +192,10 @@ def bench_static(T, args) -> dict:
     if os.path.exists(args.selector):
         sel = Lm.load_selector(args.selector)
         table, tuned, meta = sel["table"], sel["schedule"], sel["meta"]
+        how = (f"{meta['timed_tasks']} timed tasks" if "timed_tasks" in meta
+               else meta.get("costs", "?"))
         source = (f"{os.path.relpath(args.selector, ROOT)}: trained on R-MAT "
-                  f"{meta.get('train_scales')} ({meta.get('timed_tasks')} timed tasks), "
-                  f"tuned on {meta.get('tune_graph')}")
+                  f"{meta.get('train_scales')} ({how}), tuned on {meta.get('tune_graph')}")
     else:
         train = [T.rmat_graph_device(s, 16, 100 + s) for s in (18, 20, 22)]
         _, table, losses, (feat, _) = Lm.train_selector(train, samples_per_graph=4096, steps=400)

@@ -51,6 +51,7 @@ EXPORTS = [
     "tc_batched_graph_create", "tc_batched_graph_count", "tc_read_graph", "tc_host_free",
     "tc_lb_edge_prefix", "tc_graph_get_view", "tc_batch_update_directed", "tc_graph_set_growth",
     "tc_graph_compare", "tc_csr_count_device32", "tc_csr_count_device_noalloc",
+    "tc_strategy_table_set",
 ]
 
 
@@ -165,6 +166,7 @@ def lib():
         L.tc_mlp_forward.argtypes = [vp, vp, i64, vp]
         L.tc_model_load.argtypes = [vp]
         L.tc_model_table.argtypes = [vp, C.POINTER(i32)]
+        L.tc_strategy_table_set.argtypes = [vp, i32]
         L.tc_autotune.argtypes = [vp, C.POINTER(Schedule), C.POINTER(C.c_double)]
         L.tc_schedule_set_default.argtypes = [C.POINTER(Schedule)]
         L.tc_batched_graph_create.argtypes = [i32, vp, vp, vp, i64, i32, vp, C.POINTER(vp)]

@@ -42,6 +42,7 @@ int mlp_train_step(float *params, float *m, float *v, int32_t step, const float 
 int mlp_forward(const float *params, const float *feat, int64_t n, float *pred);
 int model_load(const float *params_host);
 int model_table(uint8_t *table, int32_t *dim);
+int strategy_table_set(const uint8_t *table, int32_t dim);
 int autotune(Graph &g, tc_schedule *best, double *best_ms);
 int batched_graph_create(int32_t G, const int64_t *voff, const int64_t *off, const int32_t *adj,
                          int64_t nnz, int32_t nv_total, hipStream_t s, Graph **out);
@@ -540,6 +541,13 @@ int tc_model_load(const float *params_host)
 {
     TC_GUARD({
         return tc::model_load(params_host);
+    })
+}
+
+int tc_strategy_table_set(const uint8_t *table_host, int32_t dim)
+{
+    TC_GUARD({
+        return tc::strategy_table_set(table_host, dim);
     })
 }
 

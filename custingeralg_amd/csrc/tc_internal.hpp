@@ -146,7 +146,7 @@ struct Graph {
     // streaming: per-entry "in the current batch" flags parallel to the pool,
     // per-vertex marks, and batch scratch
     DevBuf flagpool, vmark;
-    DevBuf sb[24];
+    DevBuf sb[28];
     // capacity of a row that outgrows its block: len * grow_mul_x16 / 16 +
     // grow_add (the update allocator's policy); 0: the next power of two
     int32_t grow_mul_x16 = 0;

@@ -627,6 +627,34 @@ int model_load(const float *params_host)
     return TC_OK;
 }
 
+// A strategy table given directly (an in-situ sweep's choices, or a table a
+// caller saved): entries are strategies 0..3 (merge, bsearch, hash, merge
+// path), read by POLICY_LEARNED like a model's table.
+int strategy_table_set(const uint8_t *table, int32_t dim)
+{
+    std::lock_guard<std::mutex> lk(g_learn_mu);
+    if (!table) {
+        g_learn.ready = false;
+        return TC_OK;
+    }
+    if (dim != kTableDim) {
+        set_last_error(TC_ERR_INVALID, "tc_strategy_table_set: the table must be 32 x 32");
+        return TC_ERR_INVALID;
+    }
+    for (int i = 0; i < kTableDim * kTableDim; ++i)
+        if (table[i] > 3) {
+            set_last_error(TC_ERR_INVALID, "tc_strategy_table_set: strategies are 0..3");
+            return TC_ERR_INVALID;
+        }
+    if (!g_learn.wb) {
+        TC_HIP(hipMalloc(&g_learn.wb, NP * sizeof(uint16_t)));
+        TC_HIP(hipMalloc(&g_learn.table, kTableDim * kTableDim));
+    }
+    TC_HIP(hipMemcpy(g_learn.table, table, kTableDim * kTableDim, hipMemcpyHostToDevice));
+    g_learn.ready = true;
+    return TC_OK;
+}
+
 int model_table(uint8_t *table, int32_t *dim)
 {
     std::lock_guard<std::mutex> lk(g_learn_mu);

@@ -27,6 +27,13 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kBigRow = 4096;   // rows this long are compacted / merged by a whole block
 constexpr int kRowThreads = 1024;
+constexpr int kDeltaBlocks = 2048;  // k_delta_lb's grid (4 waves per block)
+// The insertion's pool reservation, words of a device control block: the
+// pool's first free entry and capacity (set by the host per batch), whether
+// the rows that move fit (else no row is touched, and the host grows the
+// pool and runs the merges again), where their blocks start, how many
+// entries they take.
+constexpr int kCtlPoolUsed = 0, kCtlPoolCap = 1, kCtlFits = 2, kCtlBase = 3, kCtlGrow = 4;
 
 __device__ inline int64_t find_pos(const int32_t *__restrict__ pool, int64_t base, int32_t n,
                                    int32_t key)
@@ -38,6 +45,11 @@ __device__ inline int64_t find_pos(const int32_t *__restrict__ pool, int64_t bas
     }
     return (lo < n && pool[base + lo] == key) ? base + lo : -1;
 }
+
+// Bound of a grid-stride loop: n, or the count *nd the device computed when
+// one is given.  The batch sequence sizes its launches by the batch size and
+// reads the counts its own kernels produced, so it needs no host read-back.
+__device__ inline int64_t bound(int64_t n, const int64_t *nd) { return nd ? min(n, *nd) : n; }
 
 __global__ void k_batch_check(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
                               int64_t B, int32_t nv, unsigned long long *__restrict__ bad)
@@ -117,13 +129,21 @@ __global__ void k_compact_pairs(const uint64_t *__restrict__ keys, int64_t n, in
 // Locates (a,b) and (b,a) in the store; sel = 1 where the pair is present.
 // Directed: (a,b) only (pos_ba = -1).
 __global__ void k_find_pairs(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
-                             int64_t n, const int32_t *__restrict__ pool,
+                             int64_t n, const int64_t *__restrict__ nd,
+                             const int32_t *__restrict__ pool,
                              const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
                              int64_t *__restrict__ pos_ab, int64_t *__restrict__ pos_ba,
                              int32_t *__restrict__ present, int directed)
 {
+    const int64_t lim = bound(n, nd);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
+        if (i >= lim) {  // past the count: nothing present (the scans run over n)
+            pos_ab[i] = -1;
+            pos_ba[i] = -1;
+            present[i] = 0;
+            continue;
+        }
         int32_t a = pa[i], b = pb[i];
         int64_t x = find_pos(pool, boff[a], len[a], b);
         int64_t y = directed ? -1 : find_pos(pool, boff[b], len[b], a);
@@ -138,12 +158,16 @@ __global__ void k_find_pairs(const int32_t *__restrict__ pa, const int32_t *__re
 // the update -- a nonzero count means the store may no longer be symmetric.
 __global__ void k_reverse_check(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                                 const int32_t *__restrict__ sel, int64_t n,
+                                const int64_t *__restrict__ nd,
+                                const unsigned long long *__restrict__ ctl,
                                 const int32_t *__restrict__ pool, const int64_t *__restrict__ boff,
                                 const int32_t *__restrict__ len, int want,
                                 unsigned long long *__restrict__ bad)
 {
     unsigned long long c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+    // (an insertion whose merges did not run is checked when they do)
+    const int64_t lim = (ctl && !ctl[kCtlFits]) ? 0 : bound(n, nd);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (!sel[i]) continue;
         const int32_t a = pa[i], b = pb[i];
@@ -155,10 +179,11 @@ __global__ void k_reverse_check(const int32_t *__restrict__ pa, const int32_t *_
 }
 
 __global__ void k_set_flags(const int64_t *__restrict__ pos_ab, const int64_t *__restrict__ pos_ba,
-                            const int32_t *__restrict__ sel, int64_t n, uint8_t val,
-                            uint8_t *__restrict__ flag)
+                            const int32_t *__restrict__ sel, int64_t n, const int64_t *__restrict__ nd,
+                            uint8_t val, uint8_t *__restrict__ flag)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+    const int64_t lim = bound(n, nd);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (sel && !sel[i]) continue;
         if (pos_ab[i] >= 0) flag[pos_ab[i]] = val;
@@ -166,19 +191,24 @@ __global__ void k_set_flags(const int64_t *__restrict__ pos_ab, const int64_t *_
     }
 }
 
-__global__ void k_not(const int32_t *__restrict__ in, int64_t n, int32_t *__restrict__ out)
+// out = !in over the first *nd entries (n without nd), 0 after them up to n.
+__global__ void k_not(const int32_t *__restrict__ in, int64_t n, const int64_t *__restrict__ nd,
+                      int32_t *__restrict__ out)
 {
+    const int64_t lim = bound(n, nd);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = in[i] ? 0 : 1;
+        out[i] = i < lim ? (in[i] ? 0 : 1) : 0;
 }
 
 // Marks each endpoint of selected pairs once; unmarked vertices are queued.
 __global__ void k_queue_endpoints(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
-                                  const int32_t *__restrict__ sel, int64_t n,
+                                  const int32_t *__restrict__ sel, int64_t n0,
+                                  const int64_t *__restrict__ nd,
                                   int32_t *__restrict__ mark, int32_t *__restrict__ queue,
                                   unsigned long long *__restrict__ qlen)
 {
+    const int64_t n = bound(n0, nd);
     // one queue reservation per wave: appends to a single counter serialise
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -201,8 +231,10 @@ __global__ void k_queue_endpoints(const int32_t *__restrict__ pa, const int32_t 
     }
 }
 
-__global__ void k_unmark(const int32_t *__restrict__ queue, int64_t n, int32_t *__restrict__ mark)
+__global__ void k_unmark(const int32_t *__restrict__ queue, int64_t n0, const int64_t *__restrict__ nd,
+                         int32_t *__restrict__ mark)
 {
+    const int64_t n = bound(n0, nd);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         mark[queue[i]] = 0;
@@ -210,10 +242,12 @@ __global__ void k_unmark(const int32_t *__restrict__ queue, int64_t n, int32_t *
 
 // Removes flagged entries from each queued vertex's block (one wave per
 // vertex, order preserving, in place).
-__global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n,
+__global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n0,
+                               const int64_t *__restrict__ nd,
                                const int64_t *__restrict__ boff, int32_t *__restrict__ len,
                                int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
 {
+    const int64_t n = bound(n0, nd);
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -249,9 +283,11 @@ __global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n,
 // k_compact_rows for rows of at least kBigRow entries: one block per row,
 // chunks of blockDim entries, block-wide prefix of the kept entries.
 __global__ __launch_bounds__(kRowThreads) void k_compact_rows_block(
-    const int32_t *__restrict__ queue, int64_t n, const int64_t *__restrict__ boff,
+    const int32_t *__restrict__ queue, int64_t n0, const int64_t *__restrict__ nd,
+    const int64_t *__restrict__ boff,
     int32_t *__restrict__ len, int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
 {
+    const int64_t n = bound(n0, nd);
     __shared__ int32_t wsum[kRowThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
@@ -297,10 +333,12 @@ __global__ __launch_bounds__(kRowThreads) void k_compact_rows_block(
 // (one, the pair as given, for a directed batch).
 __global__ void k_directed_keys(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                                 const int64_t *__restrict__ sidx, int64_t n,
+                                const int64_t *__restrict__ nd,
                                 const int32_t *__restrict__ sel, int bv, int directed,
                                 uint64_t *__restrict__ keys)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+    const int64_t lim = bound(n, nd);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < lim;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (!sel[i]) continue;
         uint64_t a = (uint32_t)pa[i], b = (uint32_t)pb[i];
@@ -314,12 +352,38 @@ __global__ void k_directed_keys(const int32_t *__restrict__ pa, const int32_t *_
     }
 }
 
-__global__ void k_segment_heads(const uint64_t *__restrict__ keys, int64_t n, int bv,
+// Keys at or above pad are padding (sorted past every real key).
+__global__ void k_segment_heads(const uint64_t *__restrict__ keys, int64_t n, int bv, uint64_t pad,
                                 int32_t *__restrict__ head)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        head[i] = (i == 0) || ((keys[i] >> bv) != (keys[i - 1] >> bv));
+        head[i] = keys[i] < pad && ((i == 0) || ((keys[i] >> bv) != (keys[i - 1] >> bv)));
+}
+
+__global__ void k_fill_keys(uint64_t *__restrict__ keys, int64_t n, uint64_t pad)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = pad;
+}
+
+__global__ void k_nonzero(const int32_t *__restrict__ in, int64_t n, int32_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i] != 0;
+}
+
+// Device scalars: dst[i] = src[i]; dst = m * src.
+__global__ void k_copy_i64(const int64_t *__restrict__ src, int64_t *__restrict__ dst, int64_t n)
+{
+    if ((int64_t)threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
+}
+
+__global__ void k_scale_i64(const int64_t *__restrict__ src, int64_t m, int64_t *__restrict__ dst)
+{
+    if (threadIdx.x == 0) *dst = m * *src;
 }
 
 // Per affected vertex: its segment in the sorted directed keys, the new length
@@ -344,12 +408,19 @@ __global__ void k_segments(const uint64_t *__restrict__ keys, const int32_t *__r
 // Block size of each affected row that outgrows its block (0: it fits):
 // nl * mul_x16 / 16 + add when the update allocator set a policy, else the
 // next power of two; never below nl, never above INT32_MAX.
-__global__ void k_need(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg,
-                       int64_t n, const int32_t *__restrict__ len, const int32_t *__restrict__ cap,
+__global__ void k_need(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg,
+                       int64_t nseg0, const int64_t *__restrict__ nsegd, int64_t n0,
+                       const int64_t *__restrict__ nd,
+                       const int32_t *__restrict__ len, const int32_t *__restrict__ cap,
                        int32_t mul_x16, int32_t add, int32_t *__restrict__ need)
 {
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg;
+    const int64_t nseg = bound(nseg0, nsegd), n = bound(n0, nd);
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg0;
          s += (int64_t)gridDim.x * blockDim.x) {
+        if (s >= nseg) {  // past the segments: no block (the scan runs over nseg0)
+            need[s] = 0;
+            continue;
+        }
         int64_t cnt = ((s + 1 < nseg) ? sbeg[s + 1] : n) - sbeg[s];
         int64_t nl = (int64_t)len[sv[s]] + cnt;
         int64_t c = 0;
@@ -373,13 +444,18 @@ __global__ void k_need(const int32_t *__restrict__ sv, const int64_t *__restrict
 // destinations never fall below the chunk); phase 3 writes the new entries
 // into the remaining slots with flag 1 and leaves their slots in newpos.
 __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg,
-                             int64_t nseg, int64_t n, const uint64_t *__restrict__ keys, int bv,
+                             int64_t nseg0, const int64_t *__restrict__ nsegd, int64_t n0,
+                             const int64_t *__restrict__ nd,
+                             const uint64_t *__restrict__ keys, int bv,
                              const int32_t *__restrict__ need, const int64_t *__restrict__ nscan,
-                             int64_t pool_used, int64_t *__restrict__ boff,
+                             const unsigned long long *__restrict__ ctl, int64_t *__restrict__ boff,
                              int32_t *__restrict__ len, int32_t *__restrict__ cap,
                              int32_t *__restrict__ pool, uint8_t *__restrict__ flag,
                              int64_t *__restrict__ newpos)
 {
+    if (!ctl[kCtlFits]) return;
+    const int64_t nseg = bound(nseg0, nsegd), n = bound(n0, nd);
+    const int64_t pool_used = (int64_t)ctl[kCtlBase];
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t m = (1ull << bv) - 1;
@@ -464,12 +540,17 @@ __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__re
 // k_merge_rows for rows of at least kBigRow entries: the same three phases
 // with a whole block per row (block barriers between reads and writes).
 __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
-    const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg, int64_t n,
+    const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg0,
+    const int64_t *__restrict__ nsegd, int64_t n0, const int64_t *__restrict__ nd,
     const uint64_t *__restrict__ keys, int bv, const int32_t *__restrict__ need,
-    const int64_t *__restrict__ nscan, int64_t pool_used, int64_t *__restrict__ boff,
+    const int64_t *__restrict__ nscan, const unsigned long long *__restrict__ ctl,
+    int64_t *__restrict__ boff,
     int32_t *__restrict__ len, int32_t *__restrict__ cap, int32_t *__restrict__ pool,
     uint8_t *__restrict__ flag, int64_t *__restrict__ newpos)
 {
+    if (!ctl[kCtlFits]) return;
+    const int64_t nseg = bound(nseg0, nsegd), n = bound(n0, nd);
+    const int64_t pool_used = (int64_t)ctl[kCtlBase];
     const uint64_t m = (1ull << bv) - 1;
     const int32_t B = blockDim.x;
     for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
@@ -543,11 +624,44 @@ __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
     }
 }
 
-__global__ void k_clear_flags(const int64_t *__restrict__ pos, int64_t n, uint8_t *__restrict__ flag)
+__global__ void k_clear_flags(const int64_t *__restrict__ pos, int64_t n0, const int64_t *__restrict__ nd,
+                              const unsigned long long *__restrict__ ctl, uint8_t *__restrict__ flag)
 {
+    if (ctl && !ctl[kCtlFits]) return;  // the merges did not run: no new entries
+    const int64_t n = bound(n0, nd);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         flag[pos[i]] = 0;
+}
+
+// Reserves the pool room of the moved blocks: grow = nscan[n] entries from
+// the first free one, if the pool holds them.
+__global__ void k_pool_reserve(const int64_t *__restrict__ nscan, int64_t n,
+                               unsigned long long *__restrict__ ctl)
+{
+    if (threadIdx.x != 0) return;
+    const unsigned long long grow = (unsigned long long)nscan[n];
+    ctl[kCtlGrow] = grow;
+    const unsigned long long used = ctl[kCtlPoolUsed];
+    if (used + grow <= ctl[kCtlPoolCap]) {
+        ctl[kCtlFits] = 1;
+        ctl[kCtlBase] = used;
+        ctl[kCtlPoolUsed] = used + grow;
+    } else {
+        ctl[kCtlFits] = 0;
+    }
+}
+
+__global__ void k_set_ctl(unsigned long long *__restrict__ ctl, unsigned long long pool_used,
+                          unsigned long long pool_cap)
+{
+    if (threadIdx.x == 0) {
+        ctl[kCtlPoolUsed] = pool_used;
+        ctl[kCtlPoolCap] = pool_cap;
+        ctl[kCtlFits] = 0;
+        ctl[kCtlBase] = 0;
+        ctl[kCtlGrow] = 0;
+    }
 }
 
 // Delta over undirected batch pairs present in G* (sel[t] != 0): the shorter
@@ -557,13 +671,18 @@ __global__ void k_clear_flags(const int64_t *__restrict__ pos, int64_t n, uint8_
 // shorter list of every selected edge) are numbered by an exclusive prefix
 // pre[] over the edges and dealt to the waves in 64-item chunks, so an edge
 // between two hubs is spread over many waves instead of one.
+// (items 0 past the count *nd, and everywhere when the merges of this
+// insertion did not run: ctl says so)
 __global__ void k_delta_items(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                               const int32_t *__restrict__ sel, int64_t n,
+                              const int64_t *__restrict__ nd,
+                              const unsigned long long *__restrict__ ctl,
                               const int32_t *__restrict__ len, int32_t *__restrict__ items)
 {
+    const int64_t lim = (ctl && !ctl[kCtlFits]) ? 0 : bound(n, nd);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (int64_t)gridDim.x * blockDim.x)
-        items[t] = (sel == nullptr || sel[t]) ? min(len[pa[t]], len[pb[t]]) : 0;
+        items[t] = (t < lim && (sel == nullptr || sel[t])) ? min(len[pa[t]], len[pb[t]]) : 0;
 }
 
 // Largest t in [lo, hi] with pre[t] <= i.
@@ -581,12 +700,13 @@ __device__ inline int64_t edge_of(const int64_t *__restrict__ pre, int64_t i, in
 // two independent load chains in flight.
 __global__ __launch_bounds__(kThreads) void k_delta_lb(
     const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
-    const int64_t *__restrict__ pre, int64_t n, int64_t total,
+    const int64_t *__restrict__ pre, int64_t n,
     const int32_t *__restrict__ pool, const int64_t *__restrict__ boff,
     const int32_t *__restrict__ len, const uint8_t *__restrict__ flag, int32_t sign,
     uint32_t *__restrict__ out, uint32_t *__restrict__ ecnt, unsigned long long *__restrict__ gsum)
 {
     const int lane = threadIdx.x & (kWave - 1);
+    const int64_t total = pre[n];  // items over all edges (the scan's total)
     const int64_t nchunks = (total + kWave - 1) / kWave;
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     int64_t local = 0;
@@ -849,9 +969,12 @@ __global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n)
 int64_t read_i64(const int64_t *d, hipStream_t s);
 
 // Delta of the selected edges (pa[t], pb[t]) with the given sign, load
-// balanced over the items of the whole batch.
+// balanced over the items of the whole batch.  n bounds the edges; nd, when
+// given, is their count on the device and ctl (when given) switches the
+// delta off if the insertion's merges did not run.  No host read-back.
 int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel, int64_t n,
-              int32_t sign, uint32_t *out, unsigned long long *gsum)
+              const int64_t *nd, const unsigned long long *ctl, int32_t sign, uint32_t *out,
+              unsigned long long *gsum)
 {
     if (n <= 0) return TC_OK;
     hipStream_t s = g.stream;
@@ -862,19 +985,13 @@ int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel
     int64_t *pre = g.lbuf[1].as<int64_t>();
     uint32_t *ecnt = g.lbuf[2].as<uint32_t>();
     hipLaunchKernelGGL(k_delta_items, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, sel, n,
-                       g.st.len.as<int32_t>(), items);
+                       nd, ctl, g.st.len.as<int32_t>(), items);
     TC_TRY(scan_counts_i32(items, pre, n, g.tmp, s));
-    const int64_t total = read_i64(pre + n, s);
-    if (total < 0) {
-        set_last_error(TC_ERR_HIP, "delta: item count read-back failed");
-        return TC_ERR_HIP;
-    }
-    if (total == 0) return TC_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, ecnt, n);
-    const int64_t nchunks = (total + kWave - 1) / kWave;
-    hipLaunchKernelGGL(k_delta_lb, dim3(grid_for(nchunks, kThreads / kWave, 4096)), dim3(kThreads),
-                       0, s, pa, pb, pre, n, total, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
-                       g.st.len.as<int32_t>(), g.flagpool.as<uint8_t>(), sign, out, ecnt, gsum);
+    // a fixed grid: the item total is known on the device only
+    hipLaunchKernelGGL(k_delta_lb, dim3(kDeltaBlocks), dim3(kThreads), 0, s, pa, pb, pre, n,
+                       g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                       g.flagpool.as<uint8_t>(), sign, out, ecnt, gsum);
     hipLaunchKernelGGL(k_delta_edges, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, pa, pb, n, ecnt,
                        sign, out, gsum);
     TC_HIP(hipGetLastError());
@@ -925,9 +1042,9 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         return TC_ERR_INVALID;
     }
     TC_TRY(g.ensure_events());
-    TC_TRY(g.accum.ensure(16 * sizeof(unsigned long long)));
+    TC_TRY(g.accum.ensure(32 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
-    hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc, 16);
+    hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc, 32);
     hipLaunchKernelGGL(k_batch_check, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, u, v, B, nv, acc);
     TC_HIP(hipGetLastError());
     unsigned long long hb[2];
@@ -949,25 +1066,55 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     int bv = 1;
     while ((int64_t(1) << bv) < nv) ++bv;
     const int bits = 2 * bv + 1;
-    // sort (pair, op) keys and split unique inserts / deletes
+    // From here on the sequence reads back nothing: every launch is sized by
+    // the batch (B pairs, 2B directed entries) and bounded on the device by
+    // the counts the kernels before it produced (the scans' totals).  The
+    // host reads the counts once, at the end.  Only a pool too small for the
+    // rows that move sends the host back to grow it and rerun the merges.
+    const int64_t B2 = 2 * B;
+    const int64_t mult = directed ? 1 : 2;
+    const uint64_t pad = 1ull << (2 * bv);  // above every (src << bv | dst)
     uint64_t *keys = buf<uint64_t>(g, 0, B);
-    TC_NEED(keys);
+    int32_t *fi = buf<int32_t>(g, 3, B2), *fd = buf<int32_t>(g, 4, B2);
+    int64_t *pi = buf<int64_t>(g, 5, B2 + 1), *pd = buf<int64_t>(g, 6, B2 + 1);
+    int32_t *ia = buf<int32_t>(g, 7, B), *ib = buf<int32_t>(g, 8, B);
+    int32_t *da = buf<int32_t>(g, 9, B), *db = buf<int32_t>(g, 10, B);
+    int64_t *pab = buf<int64_t>(g, 11, B), *pba = buf<int64_t>(g, 12, B);
+    int32_t *pres = buf<int32_t>(g, 13, B), *queue = buf<int32_t>(g, 14, B2);
+    int64_t *cnt = buf<int64_t>(g, 15, B2 + 1);
+    int32_t *absent = buf<int32_t>(g, 16, B);
+    int64_t *sidx = buf<int64_t>(g, 17, B + 1);
+    uint64_t *dk = buf<uint64_t>(g, 18, B2);
+    int32_t *head = buf<int32_t>(g, 19, B2);
+    int64_t *hpos = buf<int64_t>(g, 20, B2 + 1);
+    int32_t *sv = buf<int32_t>(g, 21, B2), *need = buf<int32_t>(g, 22, B2);
+    int64_t *sbeg = buf<int64_t>(g, 23, B2);
+    TC_NEED(keys && fi && fd && pi && pd && ia && ib && da && db && pab && pba && pres && queue &&
+            cnt && absent && sidx && dk && head && hpos && sv && need && sbeg);
+    int64_t *nscan = cnt;               // the insertion's block sizes (after the deletions' count)
+    int32_t *moved = fi;                // rows that move (fi is not read after the split)
+    int64_t *newpos = buf<int64_t>(g, 24, B2);  // slots of the new entries
+    int64_t *mscan = buf<int64_t>(g, 25, B2 + 1);
+    TC_NEED(newpos && mscan);
+    // device counts
+    const int64_t *nI = pi + B, *nD = pd + B;  // unique insertions / deletions
+    const int64_t *nNew = sidx + B;             // insertions of absent pairs
+    const int64_t *nseg = hpos + B2;            // rows that receive entries
+    unsigned long long *ctl = acc + 16;         // pool reservation (kCtl*)
+    int64_t *ndir = reinterpret_cast<int64_t *>(acc + 24);  // directed new entries
+    int64_t *ndeleted = reinterpret_cast<int64_t *>(acc + 25);
+    int64_t *nmoved = reinterpret_cast<int64_t *>(acc + 26);
+    const int gB = grid_for(B, 256, 65536), gB2 = grid_for(B2, 256, 65536);
+
     // acc[5]: deletion requests of the raw batch
-    hipLaunchKernelGGL(k_batch_keys, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, u, v, is_del,
-                       B, bv, directed ? 1 : 0, keys, acc + 5);
+    hipLaunchKernelGGL(k_batch_keys, dim3(gB), dim3(256), 0, s, u, v, is_del, B, bv,
+                       directed ? 1 : 0, keys, acc + 5);
     TC_TRY(radix_sort_u64(keys, nullptr, B, bits, g.sb[1], g.sb[2], g.tmp, s));
-    int32_t *fi = buf<int32_t>(g, 3, B), *fd = buf<int32_t>(g, 4, B);
-    int64_t *pi = buf<int64_t>(g, 5, B + 1), *pd = buf<int64_t>(g, 6, B + 1);
-    TC_NEED(fi && fd && pi && pd);
-    hipLaunchKernelGGL(k_unique_split, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, keys, B, fi, fd);
+    hipLaunchKernelGGL(k_unique_split, dim3(gB), dim3(256), 0, s, keys, B, fi, fd);
     TC_TRY(scan_counts_i32(fi, pi, B, g.tmp, s));
     TC_TRY(scan_counts_i32(fd, pd, B, g.tmp, s));
-    const int64_t nI = read_i64(pi + B, s), nD = read_i64(pd + B, s);
-    int32_t *ia = buf<int32_t>(g, 7, nI), *ib = buf<int32_t>(g, 8, nI);
-    int32_t *da = buf<int32_t>(g, 9, nD), *db = buf<int32_t>(g, 10, nD);
-    TC_NEED(ia && ib && da && db);
-    hipLaunchKernelGGL(k_compact_pairs, dim3(grid_for(B, 256, 65536)), dim3(256), 0, s, keys, B, bv,
-                       fi, pi, fd, pd, ia, ib, da, db);
+    hipLaunchKernelGGL(k_compact_pairs, dim3(gB), dim3(256), 0, s, keys, B, bv, fi, pi, fd, pd, ia,
+                       ib, da, db);
     TC_HIP(hipGetLastError());
 
     uint32_t *out = delta;
@@ -975,155 +1122,140 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_TRY(g.tmp4.ensure((size_t)std::max(nv, 1) * sizeof(uint32_t)));
         out = g.tmp4.as<uint32_t>();
     }
-    double ms_delta = 0;
-    float ms = 0;
 
     // ---------------------------------------------------------- deletions
-    if (nD > 0) {
-        int64_t *pab = buf<int64_t>(g, 11, nD), *pba = buf<int64_t>(g, 12, nD);
-        int32_t *pres = buf<int32_t>(g, 13, nD);
-        int32_t *queue = buf<int32_t>(g, 14, 2 * nD);
-        TC_NEED(pab && pba && pres && queue);
-        hipLaunchKernelGGL(k_find_pairs, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, da, db, nD,
-                           g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                           pab, pba, pres, directed ? 1 : 0);
-        hipLaunchKernelGGL(k_set_flags, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, pab, pba,
-                           pres, nD, (uint8_t)1, g.flagpool.as<uint8_t>());
-        if (count) {
-            TC_HIP(hipEventRecord(g.ev[9], s));
-            TC_TRY(run_delta(g, da, db, pres, nD, -1, out, acc + 2));
-            TC_HIP(hipEventRecord(g.ev[10], s));
-        }
-        hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc + 4, 1);
-        hipLaunchKernelGGL(k_queue_endpoints, dim3(grid_for(nD, 256, 65536)), dim3(256), 0, s, da, db,
-                           pres, nD, g.vmark.as<int32_t>(), queue, acc + 4);
-        TC_HIP(hipGetLastError());
-        unsigned long long hq = 0;
-        TC_HIP(hipMemcpyAsync(&hq, acc + 4, sizeof(hq), hipMemcpyDeviceToHost, s));
-        TC_HIP(hipStreamSynchronize(s));
-        if (count) {
-            (void)hipEventElapsedTime(&ms, g.ev[9], g.ev[10]);
-            ms_delta += ms;
-        }
-        if (hq > 0) {
-            hipLaunchKernelGGL(k_compact_rows, dim3(grid_for((int64_t)hq, 4, 65536)), dim3(256), 0, s,
-                               queue, (int64_t)hq, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                               g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
-            hipLaunchKernelGGL(k_compact_rows_block, dim3(grid_for((int64_t)hq, 1, 1024)),
-                               dim3(kRowThreads), 0, s, queue, (int64_t)hq, g.st.boff.as<int64_t>(),
-                               g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
-                               g.flagpool.as<uint8_t>());
-            hipLaunchKernelGGL(k_unmark, dim3(grid_for((int64_t)hq, 256, 65536)), dim3(256), 0, s,
-                               queue, (int64_t)hq, g.vmark.as<int32_t>());
-            TC_HIP(hipGetLastError());
-        }
-        // deleted = present pairs
-        int64_t *cnt = buf<int64_t>(g, 15, nD + 1);
-        TC_NEED(cnt);
-        TC_TRY(scan_counts_i32(pres, cnt, nD, g.tmp, s));
-        bs.deleted = read_i64(cnt + nD, s);
-        g.st.nnz -= (directed ? 1 : 2) * bs.deleted;
-        if (track_sym)  // every deleted (a,b) needs (b,a) gone as well
-            hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(nD, 256, 4096)), dim3(256), 0, s, da, db,
-                               pres, nD, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
-                               g.st.len.as<int32_t>(), 0, acc + 6);
+    hipLaunchKernelGGL(k_find_pairs, dim3(gB), dim3(256), 0, s, da, db, B, nD, g.st.pool.as<int32_t>(),
+                       g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), pab, pba, pres,
+                       directed ? 1 : 0);
+    hipLaunchKernelGGL(k_set_flags, dim3(gB), dim3(256), 0, s, pab, pba, pres, B, nD, (uint8_t)1,
+                       g.flagpool.as<uint8_t>());
+    if (count) {
+        TC_HIP(hipEventRecord(g.ev[9], s));
+        TC_TRY(run_delta(g, da, db, pres, B, nD, nullptr, -1, out, acc + 2));
+        TC_HIP(hipEventRecord(g.ev[10], s));
     }
+    // acc[4]: the queue of rows to compact
+    hipLaunchKernelGGL(k_queue_endpoints, dim3(gB), dim3(256), 0, s, da, db, pres, B, nD,
+                       g.vmark.as<int32_t>(), queue, acc + 4);
+    const int64_t *qlen = reinterpret_cast<const int64_t *>(acc + 4);
+    hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, s, queue, B2, qlen,
+                       g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
+                       g.flagpool.as<uint8_t>());
+    hipLaunchKernelGGL(k_compact_rows_block, dim3(grid_for(B2, 1, 1024)), dim3(kRowThreads), 0, s,
+                       queue, B2, qlen, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                       g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
+    hipLaunchKernelGGL(k_unmark, dim3(gB2), dim3(256), 0, s, queue, B2, qlen, g.vmark.as<int32_t>());
+    TC_HIP(hipGetLastError());
+    // deleted = present pairs (pres is 0 past the deletions)
+    TC_TRY(scan_counts_i32(pres, cnt, B, g.tmp, s));
+    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, s, cnt + B, ndeleted, (int64_t)1);
+    if (track_sym)  // every deleted (a,b) needs (b,a) gone as well
+        hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, da, db, pres,
+                           B, nD, nullptr, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                           g.st.len.as<int32_t>(), 0, acc + 6);
 
     // ---------------------------------------------------------- insertions
-    if (nI > 0) {
-        int64_t *pab = buf<int64_t>(g, 11, nI), *pba = buf<int64_t>(g, 12, nI);
-        int32_t *pres = buf<int32_t>(g, 13, nI), *absent = buf<int32_t>(g, 16, nI);
-        int64_t *sidx = buf<int64_t>(g, 17, nI + 1);
-        TC_NEED(pab && pba && pres && absent && sidx);
-        hipLaunchKernelGGL(k_find_pairs, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, ia, ib, nI,
-                           g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                           pab, pba, pres, directed ? 1 : 0);
-        hipLaunchKernelGGL(k_not, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, pres, nI, absent);
-        TC_TRY(scan_counts_i32(absent, sidx, nI, g.tmp, s));
-        const int64_t nNew = read_i64(sidx + nI, s);
-        bs.inserted = nNew;
-        if (nNew > 0) {
-            const int64_t nd = (directed ? 1 : 2) * nNew;
-            uint64_t *dk = buf<uint64_t>(g, 18, nd);
-            int32_t *head = buf<int32_t>(g, 19, nd);
-            int64_t *hpos = buf<int64_t>(g, 20, nd + 1);
-            TC_NEED(dk && head && hpos);
-            hipLaunchKernelGGL(k_directed_keys, dim3(grid_for(nI, 256, 65536)), dim3(256), 0, s, ia, ib,
-                               sidx, nI, absent, bv, directed ? 1 : 0, dk);
-            TC_TRY(radix_sort_u64(dk, nullptr, nd, 2 * bv, g.sb[1], g.sb[2], g.tmp, s));
-            hipLaunchKernelGGL(k_segment_heads, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, dk, nd,
-                               bv, head);
-            TC_TRY(scan_counts_i32(head, hpos, nd, g.tmp, s));
-            const int64_t nseg = read_i64(hpos + nd, s);
-            int32_t *sv = buf<int32_t>(g, 21, nseg), *need = buf<int32_t>(g, 22, nseg);
-            int64_t *sbeg = buf<int64_t>(g, 23, nseg);
-            int64_t *nscan = buf<int64_t>(g, 15, nseg + 1);
-            int64_t *newpos = buf<int64_t>(g, 3, nd);  // reuses the split flags' slot
-            TC_NEED(sv && need && sbeg && nscan && newpos);
-            hipLaunchKernelGGL(k_segments, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, dk, head,
-                               hpos, nd, bv, nseg, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), sv,
-                               sbeg, need);
-            hipLaunchKernelGGL(k_need, dim3(grid_for(nseg, 256, 65536)), dim3(256), 0, s, sv, sbeg,
-                               nseg, nd, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
-                               g.grow_mul_x16, g.grow_add, need);
-            TC_TRY(scan_counts_i32(need, nscan, nseg, g.tmp, s));
-            const int64_t grow = read_i64(nscan + nseg, s);
-            TC_TRY(ensure_pool(g, grow));
-            // count moved blocks
-            {
-                int32_t *moved = buf<int32_t>(g, 4, nseg);
-                int64_t *mscan = buf<int64_t>(g, 6, nseg + 1);
-                TC_NEED(moved && mscan);
-                hipLaunchKernelGGL(k_not, dim3(grid_for(nseg, 256, 65536)), dim3(256), 0, s, need, nseg,
-                                   moved);
-                TC_TRY(scan_counts_i32(moved, mscan, nseg, g.tmp, s));
-                bs.reallocations = nseg - read_i64(mscan + nseg, s);
-            }
-            hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(nseg, 4, 65536)), dim3(256), 0, s, sv, sbeg,
-                               nseg, nd, dk, bv, need, nscan, g.st.pool_used, g.st.boff.as<int64_t>(),
-                               g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
-                               g.flagpool.as<uint8_t>(), newpos);
-            hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(nseg, 1, 1024)), dim3(kRowThreads), 0,
-                               s, sv, sbeg, nseg, nd, dk, bv, need, nscan, g.st.pool_used,
-                               g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                               g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
-                               g.flagpool.as<uint8_t>(), newpos);
-            TC_HIP(hipGetLastError());
-            g.st.pool_used += grow;
-            g.st.reallocs += bs.reallocations;
-            g.st.nnz += nd;
-            if (count) {
-                TC_HIP(hipEventRecord(g.ev[11], s));
-                TC_TRY(run_delta(g, ia, ib, absent, nI, 1, out, acc + 2));
-                TC_HIP(hipEventRecord(g.ev[12], s));
-            }
-            hipLaunchKernelGGL(k_clear_flags, dim3(grid_for(nd, 256, 65536)), dim3(256), 0, s, newpos, nd,
-                               g.flagpool.as<uint8_t>());
-            if (track_sym)  // every inserted (a,b) needs (b,a) present
-                hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(nI, 256, 4096)), dim3(256), 0, s, ia,
-                                   ib, absent, nI, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
-                                   g.st.len.as<int32_t>(), 1, acc + 6);
-            TC_HIP(hipGetLastError());
-            TC_HIP(hipStreamSynchronize(s));
-            if (count) {
-                (void)hipEventElapsedTime(&ms, g.ev[11], g.ev[12]);
-                ms_delta += ms;
-            }
+    hipLaunchKernelGGL(k_find_pairs, dim3(gB), dim3(256), 0, s, ia, ib, B, nI, g.st.pool.as<int32_t>(),
+                       g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), pab, pba, pres,
+                       directed ? 1 : 0);
+    hipLaunchKernelGGL(k_not, dim3(gB), dim3(256), 0, s, pres, B, nI, absent);
+    TC_TRY(scan_counts_i32(absent, sidx, B, g.tmp, s));
+    hipLaunchKernelGGL(k_scale_i64, dim3(1), dim3(64), 0, s, nNew, mult, ndir);
+    // directed keys of the new entries, padded to 2B, sorted
+    hipLaunchKernelGGL(k_fill_keys, dim3(gB2), dim3(256), 0, s, dk, B2, pad);
+    hipLaunchKernelGGL(k_directed_keys, dim3(gB), dim3(256), 0, s, ia, ib, sidx, B, nI, absent, bv,
+                       directed ? 1 : 0, dk);
+    TC_TRY(radix_sort_u64(dk, nullptr, B2, 2 * bv + 1, g.sb[1], g.sb[2], g.tmp, s));
+    hipLaunchKernelGGL(k_segment_heads, dim3(gB2), dim3(256), 0, s, dk, B2, bv, pad, head);
+    TC_TRY(scan_counts_i32(head, hpos, B2, g.tmp, s));
+    hipLaunchKernelGGL(k_segments, dim3(gB2), dim3(256), 0, s, dk, head, hpos, B2, bv, B2,
+                       g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), sv, sbeg, need);
+    hipLaunchKernelGGL(k_need, dim3(gB2), dim3(256), 0, s, sv, sbeg, B2, nseg, B2, ndir,
+                       g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.grow_mul_x16, g.grow_add,
+                       need);
+    TC_TRY(scan_counts_i32(need, nscan, B2, g.tmp, s));
+    // rows that move = rows with a block size
+    hipLaunchKernelGGL(k_nonzero, dim3(gB2), dim3(256), 0, s, need, B2, moved);
+    TC_TRY(scan_counts_i32(moved, mscan, B2, g.tmp, s));
+    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, s, mscan + B2, nmoved, (int64_t)1);
+    TC_HIP(hipGetLastError());
+    auto merge_tail = [&](bool first) -> int {
+        hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, s, ctl,
+                           (unsigned long long)g.st.pool_used,
+                           (unsigned long long)(g.st.pool.bytes / sizeof(int32_t)));
+        hipLaunchKernelGGL(k_pool_reserve, dim3(1), dim3(64), 0, s, nscan, B2, ctl);
+        hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, s, sv, sbeg, B2,
+                           nseg, B2, ndir, dk, bv, need, nscan, ctl, g.st.boff.as<int64_t>(),
+                           g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
+                           g.flagpool.as<uint8_t>(), newpos);
+        hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(B2, 1, 1024)), dim3(kRowThreads), 0, s, sv,
+                           sbeg, B2, nseg, B2, ndir, dk, bv, need, nscan, ctl,
+                           g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
+                           g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>(), newpos);
+        TC_HIP(hipGetLastError());
+        if (count) {
+            TC_HIP(hipEventRecord(g.ev[first ? 11 : 14], s));
+            TC_TRY(run_delta(g, ia, ib, absent, B, nI, ctl, 1, out, acc + 2));
+            TC_HIP(hipEventRecord(g.ev[first ? 12 : 15], s));
         }
-    }
+        hipLaunchKernelGGL(k_clear_flags, dim3(gB2), dim3(256), 0, s, newpos, B2, ndir, ctl,
+                           g.flagpool.as<uint8_t>());
+        if (track_sym)  // every inserted (a,b) needs (b,a) present
+            hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, ia, ib,
+                               absent, B, nI, ctl, g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(),
+                               g.st.len.as<int32_t>(), 1, acc + 6);
+        TC_HIP(hipGetLastError());
+        return TC_OK;
+    };
+    TC_TRY(merge_tail(true));
     TC_HIP(hipEventRecord(g.ev[13], s));
     TC_HIP(hipEventSynchronize(g.ev[13]));
+    // all counts at once: acc[2..6], the pool reservation, the new entries,
+    // the deletions, the moved rows
+    unsigned long long h[27];
+    TC_HIP(hipMemcpy(h, acc, sizeof(h), hipMemcpyDeviceToHost));
+    float ms = 0;
+    double ms_delta = 0;
+    if (count) {
+        (void)hipEventElapsedTime(&ms, g.ev[9], g.ev[10]);
+        ms_delta += ms;
+        (void)hipEventElapsedTime(&ms, g.ev[11], g.ev[12]);
+        ms_delta += ms;
+    }
+    if (!h[16 + kCtlFits] && h[24] > 0) {
+        // the pool could not hold the rows that move: nothing was merged;
+        // grow it and run the merges (and their delta) again
+        TC_TRY(ensure_pool(g, (int64_t)h[16 + kCtlGrow]));
+        TC_TRY(ensure_flags(g));
+        TC_TRY(merge_tail(false));
+        TC_HIP(hipEventRecord(g.ev[13], s));
+        TC_HIP(hipEventSynchronize(g.ev[13]));
+        TC_HIP(hipMemcpy(h, acc, sizeof(h), hipMemcpyDeviceToHost));
+        if (!h[16 + kCtlFits]) {
+            set_last_error(TC_ERR_OOM, "batch update: the pool did not grow enough");
+            return TC_ERR_OOM;
+        }
+        if (count) {
+            (void)hipEventElapsedTime(&ms, g.ev[14], g.ev[15]);
+            ms_delta += ms;
+        }
+    }
     (void)hipEventElapsedTime(&ms, g.ev[8], g.ev[13]);
+    const int64_t inserted = (int64_t)h[24] / mult;
+    bs.inserted = inserted;
+    bs.deleted = (int64_t)h[25];
+    bs.reallocations = inserted > 0 ? (int64_t)h[26] : 0;
+    if (inserted > 0) g.st.pool_used = (int64_t)h[16 + kCtlPoolUsed];
+    g.st.reallocs += bs.reallocations;
+    g.st.nnz += (int64_t)h[24] - mult * bs.deleted;
     bs.ms_delta = ms_delta;
     bs.ms_update = ms - ms_delta;
     // no-ops: requests (in-batch repeats included) that changed nothing
-    unsigned long long h[5] = {0, 0, 0, 0, 0};
-    TC_HIP(hipMemcpy(h, acc + 2, sizeof(h), hipMemcpyDeviceToHost));
-    const int64_t del_requests = (int64_t)h[3];
-    if (track_sym && h[4]) g.symmetric = -1;  // revalidated when next needed
+    const int64_t del_requests = (int64_t)h[5];
+    if (track_sym && h[6]) g.symmetric = -1;  // revalidated when next needed
     bs.missing_deletes = del_requests - bs.deleted;
     bs.dup_inserts = (B - del_requests) - bs.inserted;
-    if (count && dglobal) *dglobal = (int64_t)h[0];
+    if (count && dglobal) *dglobal = (int64_t)h[2];
     g.degree_stale = true;
     if (stats) *stats = bs;
     return TC_OK;

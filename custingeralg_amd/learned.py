@@ -204,5 +204,126 @@ def compare_policies(g: cuStinger, reps: int = 3):
     return out
 
 
+# ------------------------------------------------------------ in-situ training
+# The timings above are one wavefront per sampled task and cannot see what a
+# strategy costs inside a running count (lockstep lanes, per-job tiles,
+# occupancy).  The in-situ path measures whole counts instead: starting from
+# a table that encodes the fixed heuristic cell by cell, it switches one group
+# of cells to another strategy, counts, and keeps the count's time relative to
+# the unmodified table.  The model is then trained on the device on those
+# relative costs, and its BF16 table is the learned policy.
+
+# owner-tile buckets (row i = floor(log2 L) + 1) of the small and mid bins and
+# task-list buckets (column j = floor(log2 pl) + 1) the sweep varies together
+SWEEP_ROWS = ((1, 3), (4, 5), (6, 7), (8, 8), (9, 10))
+SWEEP_COLS = ((1, 3), (4, 6), (7, 9), (10, 13), (14, 31))
+TABLE_DIM = 32
+
+
+def cell_features(small_max: int = 64, mid_max: int = 512) -> np.ndarray:
+    """Features of every table cell (the device's cell_features): row i and
+    column j stand for lengths 2^(i-0.5) and 2^(j-0.5)."""
+    x = np.zeros((TABLE_DIM * TABLE_DIM, 4), np.float32)
+    for i in range(TABLE_DIM):
+        fl = 0.0 if i == 0 else i - 0.5
+        L = 2.0 ** fl
+        b = 0 if L <= small_max else (1 if L <= mid_max else 2)
+        for j in range(TABLE_DIM):
+            fp = 0.0 if j == 0 else j - 0.5
+            x[i * TABLE_DIM + j] = (fl / 16, fp / 16, (fp - fl) / 16, b / 2)
+    return x
+
+
+def fixed_table() -> np.ndarray:
+    """The fixed heuristic cell by cell (at the cell's representative
+    lengths): binary search when L * log2(pl) * 4 < pl, else hash."""
+    t = np.full((TABLE_DIM, TABLE_DIM), 2, np.uint8)
+    for i in range(1, TABLE_DIM):
+        L = 2.0 ** (i - 0.5)
+        for j in range(1, TABLE_DIM):
+            pl = 2.0 ** (j - 0.5)
+            if L * j * 4 < pl:
+                t[i, j] = 1
+    return t
+
+
+def set_table(table) -> None:
+    t = np.ascontiguousarray(table, np.uint8)
+    check(lib().tc_strategy_table_set(t.ctypes.data, TABLE_DIM))
+
+
+def _best_ms(g: cuStinger, reps: int, out) -> tuple:
+    from . import default_schedule
+    best, glob = None, None
+    for _ in range(reps):
+        _, gl, st = static_count(g, default_schedule(policy=POLICY_LEARNED), out=out)
+        glob = gl if glob is None else glob
+        if gl != glob:
+            raise RuntimeError("strategy tables disagree on the count")
+        best = st["ms_total"] if best is None else min(best, st["ms_total"])
+    return best, glob
+
+
+def insitu_sweep(g: cuStinger, reps: int = 3, strategies=(0, 1, 2)):
+    """Relative count time of every (cell group, strategy): the count with
+    that group of cells switched to the strategy over the count with the
+    fixed table (best of reps each).  Returns {(rows, cols): {s: rel}}."""
+    base = fixed_table()
+    out = DeviceArray(g.nv, np.uint32)
+    try:
+        set_table(base)
+        t0, glob0 = _best_ms(g, reps, out)
+        res = {}
+        for rows in SWEEP_ROWS:
+            for cols in SWEEP_COLS:
+                rel = {}
+                for st in strategies:
+                    t = base.copy()
+                    t[rows[0]:rows[1] + 1, cols[0]:cols[1] + 1] = st
+                    if np.array_equal(t, base):
+                        rel[st] = 1.0
+                        continue
+                    set_table(t)
+                    ms, glob = _best_ms(g, reps, out)
+                    if glob != glob0:
+                        raise RuntimeError("a strategy table changed the count")
+                    rel[st] = ms / t0
+                res[(rows, cols)] = rel
+        return res
+    finally:
+        out.free()
+        unload()
+
+
+def train_insitu(graphs, reps: int = 3, steps: int = 400, lr: float = 0.01, seed: int = 0,
+                 min_gain: float = 0.01):
+    """In-situ sweeps on each graph, then the model trained on the device on
+    the relative costs and loaded.  A group keeps the fixed heuristic's
+    strategy unless another one saved at least min_gain of the count on
+    average.  Returns (model, table, per-graph sweeps)."""
+    sweeps = [insitu_sweep(g, reps) for g in graphs]
+    base = fixed_table()
+    x = cell_features()
+    # every cell: the fixed strategy cheapest by a margin unless measured otherwise
+    tgt = np.full((TABLE_DIM * TABLE_DIM, 3), 0.5, np.float32)
+    tgt[np.arange(TABLE_DIM * TABLE_DIM), base.ravel()] = 0.0
+    for key in sweeps[0]:
+        rows, cols = key
+        rel = {st: float(np.mean([sw[key][st] for sw in sweeps])) for st in sweeps[0][key]}
+        b = int(base[rows[0], cols[0]])
+        best = min(rel, key=rel.get)
+        if rel[best] > rel.get(b, 1.0) - min_gain:
+            best = b
+        for i in range(rows[0], rows[1] + 1):
+            for j in range(cols[0], cols[1] + 1):
+                c = i * TABLE_DIM + j
+                for st, r in rel.items():
+                    tgt[c, st] = 0.0 if st == best else max(10.0 * (r - rel[best]), 0.1)
+    model = CostModel(seed=seed)
+    losses = model.fit(x, tgt, steps, lr)
+    table = model.load()
+    return model, table, losses, sweeps
+
+
 def default_model_path() -> str:
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_model.json")

@@ -402,6 +402,11 @@ int tc_mlp_forward(const float *params_dev, const float *feat_dev, int64_t n, fl
  * used by TC_POLICY_LEARNED. */
 int tc_model_load(const float *params_host);
 int tc_model_table(uint8_t *table_host, int32_t *dim);
+/* A 32 x 32 strategy table given directly (row: floor(log2 |owner tile|)+1,
+ * column: floor(log2 |task list|)+1; entries 0 merge, 1 bsearch, 2 hash,
+ * 3 merge path), used by TC_POLICY_LEARNED in place of a model's table
+ * (the in-situ sweep's probes set one per run).  NULL unloads it. */
+int tc_strategy_table_set(const uint8_t *table_host, int32_t dim);
 
 /* -------------------------------------------------------- autotuning -- */
 /* Sweeps bin thresholds and job grain on g and stores the best schedule as

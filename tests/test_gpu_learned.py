@@ -111,3 +111,48 @@ def test_autotune_keeps_counts():
     pv, glob, _ = T.static_count(g, T.default_schedule())
     assert np.array_equal(pv, O.custinger_count(off, adj))
     g.close()
+
+
+def test_insitu_training_on_held_out_graph():
+    """The selector trained in situ (whole counts with one group of table
+    cells switched at a time) on R-MAT 18 and 20, then compared with the fixed
+    heuristic on a held-out R-MAT 22 (another seed): equal counts, and the
+    learned table no more than 3% slower (best of 5 counts each)."""
+    train = [T.rmat_graph_device(s, 16, 300 + s) for s in (18, 20)]
+    try:
+        model, table, losses, sweeps = Lm.train_insitu(train, reps=2, steps=300)
+    finally:
+        for g in train:
+            g.close()
+    assert losses[-1] < losses[0]
+    assert set(np.unique(table)) <= {0, 1, 2}
+    g = T.rmat_graph_device(22, 16, 3)
+    try:
+        res = Lm.compare_policies(g, reps=5)  # asserts equal counts
+        ratio = res["fixed"]["ms_total"] / res["learned"]["ms_total"]
+        print(f"held-out R-MAT 22: fixed {res['fixed']['ms_total']:.2f} ms, learned "
+              f"{res['learned']['ms_total']:.2f} ms, learned/fixed speed {ratio:.3f}", flush=True)
+        assert res["learned"]["ms_total"] <= 1.03 * res["fixed"]["ms_total"]
+    finally:
+        g.close()
+        Lm.unload()
+
+
+def test_strategy_table_set_drives_the_policy():
+    """A table given directly: every cell merge path, then every cell binary
+    search; the counts stay the oracle's and the strategy counters follow the
+    table in the small and mid bins."""
+    off, adj = O.rmat_graph(13, 16, 8)
+    ref = O.custinger_count(off, adj)
+    g = T.cuStinger(off, adj)
+    try:
+        for s, key in ((3, "mergepath"), (1, "bsearch")):
+            Lm.set_table(np.full((32, 32), s, np.uint8))
+            pv, glob, st = T.static_count(g, T.default_schedule(policy=T.POLICY_LEARNED))
+            assert np.array_equal(pv, ref)
+            assert st["strategy_tasks"][key] > 0
+        with pytest.raises(T.TCError):
+            Lm.set_table(np.full((32, 32), 4, np.uint8))
+    finally:
+        Lm.unload()
+        g.close()
