@@ -302,6 +302,9 @@ def bench_stream(T, args) -> dict:
     sizes = W.STREAM_SIZES
     er, rmat = _device_pairs(T)
     gd = golden("stream")
+    # batches padded to size classes and replayed as hipGraphs (the first
+    # batch of a class runs plainly, the second records)
+    sched = T.default_schedule(capture=0 if args.no_capture else 1)
     per_size = {}
     t_all, n_all = 0.0, 0
     for i, B in enumerate(sizes):
@@ -314,7 +317,7 @@ def bench_stream(T, args) -> dict:
         T.synchronize()
         t0 = time.perf_counter()
         T.check(T.lib().tc_batch_apply(g.h, du.ptr, dv.ptr, dd.ptr, len(u), delta.ptr,
-                                       C.byref(dg), None, C.byref(st)))
+                                       C.byref(dg), C.byref(sched), C.byref(st)))
         T.synchronize()
         dt = time.perf_counter() - t0
         total += dg.value
@@ -324,18 +327,24 @@ def bench_stream(T, args) -> dict:
         if gd and total != 6 * gd["batches"][i]["T"]:
             raise RuntimeError(f"batch {i}: running total {total} != the oracle's "
                                f"{6 * gd['batches'][i]['T']}")
-        s = per_size.setdefault(B, {"pairs": 0, "s": 0.0, "ms_update": 0.0, "ms_delta": 0.0})
+        s = per_size.setdefault(B, {"pairs": 0, "s": 0.0, "ms_update": 0.0, "ms_delta": 0.0,
+                                    "wall_ms": [], "replayed": 0})
         s["pairs"] += len(u)
         s["s"] += dt
+        s["wall_ms"].append(dt * 1e3)
+        s["replayed"] += st.replayed
         s["ms_update"] += st.ms_update
         s["ms_delta"] += st.ms_delta
         t_all += dt
         n_all += len(u)
         log(f"[bench] batch {i}: B={len(u)} ({int(dl.sum())} deletions) {dt * 1e3:.2f} ms "
-            f"(update {st.ms_update:.2f} delta {st.ms_delta:.2f}) delta={dg.value} ok")
+            f"(update {st.ms_update:.2f} delta {st.ms_delta:.2f}{' replayed' if st.replayed else ''}) "
+            f"delta={dg.value} ok")
     g.close()
     rates = {str(B): {"updates_per_s": s["pairs"] / s["s"],
-                      "ms_update": s["ms_update"] / 5, "ms_delta": s["ms_delta"] / 5}
+                      "ms_update": s["ms_update"] / 5, "ms_delta": s["ms_delta"] / 5,
+                      "wall_ms_per_batch": [round(x, 3) for x in s["wall_ms"]],
+                      "replayed_batches": s["replayed"]}
              for B, s in per_size.items()}
     return {"metric": "streaming TC updates/sec (exact delta, checked by recount)",
             "value": n_all / t_all, "unit": "updates/s", "ms_per_step": t_all / len(sizes) * 1e3,
@@ -343,6 +352,7 @@ def bench_stream(T, args) -> dict:
             "config": {"workload": f"streaming TC, R-MAT scale {scale} ef {ef} seed {seed} base, "
                                    "20 batches 5x10^3..5x10^6, 80% inserts / 20% deletes",
                        "per_batch_size": rates, "final_global_count": total,
+                       "recorded_batches": not args.no_capture,
                        "parity": ("running total equals a GPU recount and the oracle's T "
                                   "after every batch" if gd else
                                   "running total equals a GPU recount after every batch")},
@@ -449,6 +459,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-capture", action="store_true",
+                    help="stream workload: no size classes, no recorded batches")
     ap.add_argument("--selector", default=os.path.join(ROOT, "custingeralg_amd",
                                                        "selector_model.json"))
     args = ap.parse_args()

@@ -106,7 +106,7 @@ class BatchStats(C.Structure):
     _fields_ = [("inserted", C.c_int64), ("deleted", C.c_int64), ("dup_inserts", C.c_int64),
                 ("missing_deletes", C.c_int64), ("reallocations", C.c_int64),
                 ("ms_update", C.c_double), ("ms_delta", C.c_double),
-                ("reserved", C.c_int32 * 8)]
+                ("replayed", C.c_int32), ("reserved", C.c_int32 * 7)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -532,6 +532,8 @@ void graph_destroy(Graph *g)
     if (g->events_ready)
         for (auto &e : g->ev) (void)hipEventDestroy(e);
     g->drop_capture();
+    if (g->bexec) (void)hipGraphExecDestroy(g->bexec);
+    if (g->bgraph) (void)hipGraphDestroy(g->bgraph);
     if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
     delete g;
 }

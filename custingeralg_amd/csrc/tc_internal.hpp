@@ -185,6 +185,13 @@ struct Graph {
     hipGraph_t cgraph = nullptr;
     hipGraphExec_t cexec = nullptr;
     uint64_t cexec_key = 0;
+    // recorded batch sequences (tc_batch_apply with tc_schedule.capture):
+    // one per size class and buffer layout, keyed like the counts
+    hipGraph_t bgraph = nullptr;
+    hipGraphExec_t bexec = nullptr;
+    uint64_t bexec_key = 0, bplan_key = 0;
+    bool bplan_valid = false;
+    bool bcap_failed = false;  // a recording failed: batches run plainly
     void drop_capture();
     void touch()  // the store changed
     {

@@ -241,10 +241,51 @@ __global__ void k_radix_scatter(const uint64_t *__restrict__ keys_in,
 
 }  // namespace
 
+namespace {
+
+// Keys-only sorts of at most kSmallSort keys: one block, a bitonic network in
+// LDS over the next power of two (padded with ~0, past every key).  One
+// launch instead of three per 8-bit radix pass: small update batches sort
+// twice, and their launches, not their keys, are what they cost.
+constexpr int kSmallSort = 8192;
+
+__global__ __launch_bounds__(1024) void k_sort_small(uint64_t *__restrict__ keys, int64_t n)
+{
+    __shared__ uint64_t sk[kSmallSort];
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) sk[i] = i < n ? keys[i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = sk[i], b = sk[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        sk[i] = b;
+                        sk[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) keys[i] = sk[i];
+}
+
+}  // namespace
+
 int radix_sort_u64(uint64_t *keys, int32_t *vals, int64_t n, int bits, DevBuf &alt_keys,
                    DevBuf &alt_vals, DevBuf &tmp, hipStream_t s)
 {
     if (n <= 1 || bits <= 0) return TC_OK;
+    if (!vals && n <= kSmallSort) {  // (keys below 2^bits: order by the whole key)
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, s, keys, n);
+        TC_HIP(hipGetLastError());
+        return TC_OK;
+    }
     int64_t nblocks = ceil_div(n, kSortTile);
     TC_TRY(alt_keys.ensure((size_t)n * sizeof(uint64_t)));
     if (vals) TC_TRY(alt_vals.ensure((size_t)n * sizeof(int32_t)));

@@ -221,8 +221,9 @@ typedef struct {
     int64_t missing_deletes; /* no-op deletes (absent pair or in-batch repeat)    */
     int64_t reallocations;   /* blocks moved to grow                              */
     double ms_update;        /* store update time                                 */
-    double ms_delta;         /* delta counting time                               */
-    int32_t reserved[8];
+    double ms_delta;         /* delta counting time (0 in a replay: not timed)     */
+    int32_t replayed;        /* 1: the batch ran as a recorded hipGraph            */
+    int32_t reserved[7];
 } tc_batch_stats;
 
 /* Streaming batch: cuStinger edgeInsertionsSorted / edgeDeletions
@@ -239,6 +240,11 @@ typedef struct {
 int tc_batch_apply(tc_graph *g, const int32_t *u, const int32_t *v, const uint8_t *is_delete,
                    int64_t B, uint32_t *per_vertex_delta_dev, int64_t *delta_global_host,
                    const tc_schedule *sched, tc_batch_stats *stats);
+/* With sched->capture, the batch is padded to its size class (256, then
+ * steps of an eighth of the next power of two); the second batch of a class
+ * (same output and buffers) records the sequence after validation as a
+ * hipGraph and later batches of the class replay it.  The host then reads
+ * back only twice per batch: the validation and the results. */
 
 /* Store-only update (no counting): cuStinger edgeInsertions / edgeDeletions
  * of undirected pairs (both directions of each pair) on a symmetric store. */

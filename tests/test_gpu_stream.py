@@ -301,3 +301,48 @@ def test_hub_rows_merge_and_compact(slack):
         assert np.array_equal(base + delta.to_host(), now), f"batch {bi} ({plan}): per-vertex"
         assert total == int(now.astype(np.uint64).sum()), f"batch {bi} ({plan}): global"
     g.close()
+
+
+@pytest.mark.parametrize("slack", [0, 50])
+def test_recorded_batches(slack):
+    """tc_schedule.capture: batches padded to their size class; the second
+    batch of a class records the sequence, later ones replay it.  After every
+    batch the store equals the oracle's and a plainly updated twin store
+    (tc_graph_compare), and the per-vertex delta equals the oracle's count
+    difference; the pool also outgrows its headroom (slack 0), which sends a
+    recorded batch back to the host to grow it."""
+    off, adj = O.rmat_graph(13, 8, 17)
+    n = len(off) - 1
+    rng = np.random.default_rng(23)
+    g = T.cuStinger(off, adj, slack_pct=slack)
+    twin = T.cuStinger(off, adj, slack_pct=slack)
+    delta = T.DeviceArray(n, np.uint32).zero()
+    dtwin = T.DeviceArray(n, np.uint32).zero()
+    base = O.custinger_count(off, adj)
+    sched = T.default_schedule(capture=1)
+    cur_off, cur_adj = off, adj
+    replayed = []
+    try:
+        for bi in range(12):
+            # 900..1000 pairs: one size class (1024); hub-heavy R-MAT inserts
+            bu, bv, dl = _batch(rng, n, cur_off, cur_adj, 800 + int(rng.integers(0, 50)),
+                                100 + int(rng.integers(0, 50)), rmat_scale=13 if bi % 2 else None)
+            dg, st = T.batch_apply(g, bu, bv, dl, delta, schedule=sched)
+            dg2, st2 = T.batch_apply(twin, bu, bv, dl, dtwin)
+            replayed.append(st["replayed"])
+            assert dg == dg2 and st["inserted"] == st2["inserted"] and st["deleted"] == st2["deleted"]
+            assert st["dup_inserts"] == st2["dup_inserts"]
+            assert st["missing_deletes"] == st2["missing_deletes"]
+            cur_off, cur_adj = _host_apply(n, cur_off, cur_adj, bu, bv, dl)
+            got_off, got_adj = g.to_csr()
+            assert np.array_equal(got_off, cur_off) and np.array_equal(got_adj, cur_adj), bi
+            assert g.compare(twin) == {"size_mismatch": 0, "entry_mismatch": 0, "first_vertex": -1}
+            now = O.custinger_count(cur_off, cur_adj)
+            assert np.array_equal(base + delta.to_host(), now), f"batch {bi}: per-vertex"
+            assert np.array_equal(delta.to_host(), dtwin.to_host())
+        assert replayed[0] == 0 and sum(replayed) >= 6, replayed
+    finally:
+        for x in (delta, dtwin):
+            x.free()
+        g.close()
+        twin.close()
