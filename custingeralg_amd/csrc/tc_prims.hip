@@ -104,6 +104,48 @@ __global__ void k_tile_scan(const T *__restrict__ in, int64_t *__restrict__ out,
 
 __global__ void k_zero_total(int64_t *total) { *total = 0; }
 
+// Scans of at most kScanSmall items: one block, thread t holds items
+// t + 1024k (every load and store is a full coalesced row), four wave scans
+// and one barrier.  Small update batches scan a dozen such arrays, where a
+// launch's latency is all it costs.
+constexpr int kScanSmallThreads = 1024, kScanSmallRows = 4;
+constexpr int kScanSmall = kScanSmallThreads * kScanSmallRows;
+
+template <class T>
+__global__ __launch_bounds__(kScanSmallThreads) void k_scan_small(const T *__restrict__ in,
+                                                                  int64_t *__restrict__ out, int64_t n,
+                                                                  int64_t *__restrict__ total)
+{
+    __shared__ int64_t ws[kScanSmallRows][kScanSmallThreads / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    int64_t v[kScanSmallRows], incl[kScanSmallRows];
+#pragma unroll
+    for (int k = 0; k < kScanSmallRows; ++k) {
+        const int64_t i = (int64_t)k * kScanSmallThreads + t;
+        v[k] = i < n ? (int64_t)in[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanSmallRows; ++k) {
+        incl[k] = wave_incl_scan(v[k]);
+        if (lane == kWave - 1) ws[k][wid] = incl[k];
+    }
+    __syncthreads();
+    int64_t base = 0;
+#pragma unroll
+    for (int k = 0; k < kScanSmallRows; ++k) {
+        int64_t before = 0, row = 0;
+        for (int w = 0; w < kScanSmallThreads / kWave; ++w) {
+            const int64_t x = ws[k][w];
+            before += w < wid ? x : 0;
+            row += x;
+        }
+        const int64_t i = (int64_t)k * kScanSmallThreads + t;
+        if (i < n) out[i] = base + before + incl[k] - v[k];
+        base += row;
+    }
+    if (total && t == 0) *total = base;
+}
+
 template <class T>
 int scan_impl(const T *in, int64_t *out, int64_t n, int64_t *total, DevBuf &tmp, hipStream_t s,
               size_t tmp_base)
@@ -111,6 +153,11 @@ int scan_impl(const T *in, int64_t *out, int64_t n, int64_t *total, DevBuf &tmp,
     if (n <= 0) {
         // a kernel, not a memset: counts may be recorded into a hipGraph
         if (total) hipLaunchKernelGGL(k_zero_total, dim3(1), dim3(1), 0, s, total);
+        TC_HIP(hipGetLastError());
+        return TC_OK;
+    }
+    if (n <= kScanSmall) {  // (in place too: a thread writes only what it read)
+        hipLaunchKernelGGL(k_scan_small<T>, dim3(1), dim3(kScanSmallThreads), 0, s, in, out, n, total);
         TC_HIP(hipGetLastError());
         return TC_OK;
     }
@@ -275,13 +322,72 @@ __global__ __launch_bounds__(1024) void k_sort_small(uint64_t *__restrict__ keys
     for (int i = threadIdx.x; i < n; i += blockDim.x) keys[i] = sk[i];
 }
 
+// Keys-only sorts of at most kRegSort keys: the same bitonic network with the
+// keys in registers, two per lane (wave w holds positions 128w + lane and
+// 128w + 64 + lane).  Partners closer than 64 are exchanged by shuffles,
+// those 64 apart are the lane's own pair, and only the 10 steps with partners
+// 128 or more apart go through LDS (double-buffered: one barrier each).
+constexpr int kRegSort = 2048;
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t x, int j)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, j, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), j, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The value position i keeps at step (k, j) of the network, its partner p.
+__device__ inline uint64_t bitonic_keep(int i, int j, int k, uint64_t v, uint64_t p)
+{
+    const bool asc = (i & k) == 0, lower = (i & j) == 0;
+    return lower == asc ? (v < p ? v : p) : (v < p ? p : v);
+}
+
+__global__ __launch_bounds__(1024) void k_sort_reg(uint64_t *__restrict__ keys, int64_t n)
+{
+    __shared__ uint64_t buf[2][kRegSort];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 128 * w + lane, i1 = i0 + 64;
+    uint64_t v0 = i0 < n ? keys[i0] : ~0ull, v1 = i1 < n ? keys[i1] : ~0ull;
+    int par = 0;
+    for (int k = 2; k <= kRegSort; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 128) {
+                buf[par][i0] = v0;
+                buf[par][i1] = v1;
+                __syncthreads();
+                const uint64_t p0 = buf[par][i0 ^ j], p1 = buf[par][i1 ^ j];
+                v0 = bitonic_keep(i0, j, k, v0, p0);
+                v1 = bitonic_keep(i1, j, k, v1, p1);
+                par ^= 1;
+            } else if (j == 64) {
+                const uint64_t lo = v0 < v1 ? v0 : v1, hi = v0 < v1 ? v1 : v0;
+                const bool asc = (i0 & k) == 0;
+                v0 = asc ? lo : hi;
+                v1 = asc ? hi : lo;
+            } else {
+                const uint64_t p0 = shfl_xor_u64(v0, j), p1 = shfl_xor_u64(v1, j);
+                v0 = bitonic_keep(i0, j, k, v0, p0);
+                v1 = bitonic_keep(i1, j, k, v1, p1);
+            }
+        }
+    }
+    if (i0 < n) keys[i0] = v0;
+    if (i1 < n) keys[i1] = v1;
+}
+
 }  // namespace
 
 int radix_sort_u64(uint64_t *keys, int32_t *vals, int64_t n, int bits, DevBuf &alt_keys,
                    DevBuf &alt_vals, DevBuf &tmp, hipStream_t s)
 {
     if (n <= 1 || bits <= 0) return TC_OK;
-    if (!vals && n <= kSmallSort) {  // (keys below 2^bits: order by the whole key)
+    if (!vals && n <= kRegSort) {  // (keys below 2^bits: order by the whole key)
+        hipLaunchKernelGGL(k_sort_reg, dim3(1), dim3(1024), 0, s, keys, n);
+        TC_HIP(hipGetLastError());
+        return TC_OK;
+    }
+    if (!vals && n <= kSmallSort) {
         hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, s, keys, n);
         TC_HIP(hipGetLastError());
         return TC_OK;

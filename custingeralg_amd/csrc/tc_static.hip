@@ -715,6 +715,445 @@ __global__ void k_check_cursor(const int32_t *__restrict__ cursor, const VInfo *
         atomicOr(bad, 1);
 }
 
+// ---------------------------------------------- partitioned transpose --
+//
+// The task list of owner w is [toff[w], toff[w+1]) of the task array, so a
+// run of consecutive owners owns one contiguous region.  Instead of one
+// random 8-byte append (and one cursor atomic) per plus entry, the records
+// reach their owners in three passes that each write whole runs:
+//  1. coarse: every tile of kTpTile plus entries is split in LDS by coarse
+//     bucket (a run of owners); one atomic per (tile, bucket) reserves the
+//     bucket's run in its region of a staging array (record and owner), and
+//     the tile writes its runs contiguously;
+//  2. fine: every tile of a coarse region is split the same way into the fine
+//     buckets of that coarse bucket, written into the task array itself (and
+//     the owners beside it);
+//  3. local: a fine bucket of several owners (fewer than 2 * kTpWin tasks) is
+//     counting-sorted by owner in LDS and written back in place.  A fine
+//     bucket of one owner is final after pass 2.
+// Fine buckets are the owners whose task lists start in one window of kTpWin
+// tasks, except that an owner of more than kTpWin tasks is a bucket of its
+// own; a coarse bucket is a run of G consecutive fine buckets (G about the
+// square root of their number, so that both splits have at most kTpMaxB
+// buckets).  The order of an owner's tasks is arbitrary, as with appends.
+// Hubs (ranks from sc[1] up) keep the row-built lists of
+// k_orient_transpose_hubs.
+#ifndef TC_TP_TILE
+#define TC_TP_TILE 8192
+#endif
+#ifndef TC_TP_THREADS
+#define TC_TP_THREADS 1024
+#endif
+#ifndef TC_TP_MAXB
+#define TC_TP_MAXB 2048
+#endif
+constexpr int kTpWin = 4096;
+constexpr int kTpTile = TC_TP_TILE;
+constexpr int kTpThreads = TC_TP_THREADS;
+constexpr int kTpPer = kTpTile / kTpThreads;
+constexpr int kTpMaxB = TC_TP_MAXB;
+static_assert(kTpMaxB <= 2 * kTpThreads && kTpMaxB <= 2048, "two buckets per thread in the scans");
+constexpr int kTpLocal = 2 * kTpWin;
+constexpr int64_t kTpDrop = INT64_MIN;
+
+// Fine-bucket starts: owner w < hub_lo starts a bucket when it is the first,
+// holds more than kTpWin tasks, follows such an owner, or starts its list in
+// another window than the owner before it.
+__global__ void k_tp_flags(const int64_t *__restrict__ toff, const unsigned long long *__restrict__ sc,
+                           int32_t nv, int32_t *__restrict__ flag)
+{
+    const int64_t hub_lo = (int64_t)min(sc[1], (unsigned long long)nv);
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nv;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        int32_t f = 0;
+        if (w < hub_lo) {
+            const int64_t t0 = toff[w];
+            f = w == 0 || toff[w + 1] - t0 > kTpWin;
+            if (!f) {
+                const int64_t tb = toff[w - 1];
+                f = t0 - tb > kTpWin || t0 / kTpWin != tb / kTpWin;
+            }
+        }
+        flag[w] = f;
+    }
+}
+
+// fb[i] = first owner of fine bucket i; fb[F] = hub_lo.
+__global__ void k_tp_bounds(const int32_t *__restrict__ flag, const int64_t *__restrict__ fpos,
+                            const unsigned long long *__restrict__ sc, int32_t nv,
+                            int32_t *__restrict__ fb)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t w = gid; w < nv; w += (int64_t)gridDim.x * blockDim.x)
+        if (flag[w]) fb[fpos[w]] = (int32_t)w;
+    if (gid == 0) fb[fpos[nv]] = (int32_t)min(sc[1], (unsigned long long)nv);
+}
+
+// One block of 1024 threads: ctl = {F, G, K (coarse buckets), pass-2 tiles},
+// tiles[k] = first pass-2 tile of coarse bucket k (tiles[K] = their total),
+// and the coarse cursors zeroed.
+__global__ __launch_bounds__(1024) void k_tp_plan(const int64_t *__restrict__ fpos, int32_t nv,
+                                                  const int32_t *__restrict__ fb,
+                                                  const int64_t *__restrict__ toff,
+                                                  int64_t *__restrict__ ctl, int32_t *__restrict__ tiles,
+                                                  unsigned long long *__restrict__ ccur)
+{
+    __shared__ int32_t wsum[1024 / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    const int64_t F = fpos[nv];
+    int64_t G = 1;
+    if (F > 0) {
+        G = max<int64_t>(1, (int64_t)ceil(sqrt((double)F)));
+        if (ceil_div(F, G) > kTpMaxB) G = ceil_div(F, kTpMaxB);
+    }
+    const int64_t K = F > 0 ? ceil_div(F, G) : 0;
+    int32_t c[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int64_t k = 2 * t + q;
+        c[q] = 0;
+        if (k < K) {
+            const int64_t size = toff[fb[min(G * (k + 1), F)]] - toff[fb[G * k]];
+            c[q] = (int32_t)ceil_div(size, kTpTile);
+        }
+        if (k < kTpMaxB) ccur[k] = 0ull;
+    }
+    const int32_t sum = c[0] + c[1];
+    const int32_t incl = wave_incl_scan_i32(sum);
+    if (lane == kWave - 1) wsum[wid] = incl;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+    for (int k = 0; k < 1024 / kWave; ++k) {
+        before += k < wid ? wsum[k] : 0;
+        total += wsum[k];
+    }
+    const int32_t e0 = before + incl - sum;
+    if (2 * t < K) tiles[2 * t] = e0;
+    if (2 * t + 1 < K) tiles[2 * t + 1] = e0 + c[0];
+    if (t == 0) {
+        tiles[K] = total;
+        ctl[0] = F;
+        ctl[1] = G;
+        ctl[2] = K;
+        ctl[3] = total;
+    }
+}
+
+struct TpShared {
+    int32_t bnd[kTpMaxB + 1];  // owner bounds of the tile's buckets
+    int32_t cnt[kTpMaxB];      // records per bucket, then their local offsets
+    int64_t dst[kTpMaxB];      // where the bucket's run goes, less its local offset
+    alignas(16) uint64_t rec[kTpTile];
+    alignas(16) int32_t own[kTpTile];  // (read as int4 by k_tp_coarse)
+    uint16_t bk[kTpTile];
+    int32_t wsum[kTpThreads / kWave];
+};
+
+// For every record k with bk[k] >= 0: the largest b in [0, nb) with
+// bnd[b] <= own[k] (0 when none), into bk[k].  The searches of a thread's
+// records advance in lockstep, so their LDS reads are in flight together.
+__device__ inline void tp_buckets(const int32_t *bnd, int32_t nb, const int32_t (&own)[kTpPer],
+                                  int32_t (&bk)[kTpPer])
+{
+    int32_t pos[kTpPer];
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k) pos[k] = 0;
+    int32_t step = 1;
+    while (2 * step < nb) step *= 2;
+    for (; step > 0; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < kTpPer; ++k) {
+            const int32_t q = pos[k] + step;
+            if (q < nb && bnd[q] <= own[k]) pos[k] = q;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k)
+        if (bk[k] >= 0) bk[k] = pos[k];
+}
+
+// The split of one tile: every thread holds up to kTpPer records with their
+// bucket (bk < 0: none) and rank in it (from atomics on S.cnt, complete on
+// entry up to the barrier below).  Reserves each bucket's run with one
+// atomic on cur[b], whose region is [toff[bnd[b]], toff[bnd[b+1]]); a run
+// past its region is dropped and flags the rows as not symmetric.
+__device__ inline void tp_split(TpShared &S, int32_t nb, const uint64_t (&rec)[kTpPer],
+                                const int32_t (&own)[kTpPer], const int32_t (&bk)[kTpPer],
+                                const int32_t (&rank)[kTpPer], const int64_t *__restrict__ toff,
+                                unsigned long long *__restrict__ cur, uint64_t *__restrict__ rec_out,
+                                int32_t *__restrict__ own_out, int32_t *__restrict__ bad)
+{
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    __syncthreads();  // the counts are complete
+    int32_t c[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) c[q] = 2 * t + q < nb ? S.cnt[2 * t + q] : 0;
+    const int32_t sum = c[0] + c[1];
+    const int32_t incl = wave_incl_scan_i32(sum);
+    if (lane == kWave - 1) S.wsum[wid] = incl;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+    for (int k = 0; k < kTpThreads / kWave; ++k) {
+        before += k < wid ? S.wsum[k] : 0;
+        total += S.wsum[k];
+    }
+    int32_t off = before + incl - sum;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int32_t b = 2 * t + q;
+        if (b < nb) {
+            if (c[q] > 0) {
+                const int64_t start = toff[S.bnd[b]], size = toff[S.bnd[b + 1]] - start;
+                const unsigned long long r = atomicAdd(&cur[b], (unsigned long long)c[q]);
+                if ((int64_t)r + c[q] <= size) {
+                    S.dst[b] = start + (int64_t)r - off;
+                } else {
+                    S.dst[b] = kTpDrop;
+                    atomicOr(bad, 1);  // more records than the owners' tasks
+                }
+            }
+            S.cnt[b] = off;
+        }
+        off += c[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k) {
+        if (bk[k] < 0) continue;
+        const int32_t p = S.cnt[bk[k]] + rank[k];
+        S.rec[p] = rec[k];
+        S.own[p] = own[k];
+        S.bk[p] = (uint16_t)bk[k];
+    }
+    __syncthreads();
+    for (int32_t p = t; p < total; p += kTpThreads) {
+        const int64_t d = S.dst[S.bk[p]];
+        if (d != kTpDrop) {
+            rec_out[d + p] = S.rec[p];
+            own_out[d + p] = S.own[p];
+        }
+    }
+}
+
+// Pass 1: the plus entries of one tile into their coarse buckets' regions of
+// the staging arrays (task record and owner).  Entries that point to hubs
+// are counted in sc[2] instead.  An entry's suffix length needs the end of
+// its row: the rows that end inside the tile mark their last entry (in the
+// owner staging area, free until the split), a suffix minimum spreads each
+// mark over its row, and the entries of a row that runs past the tile take
+// that row's end.
+__global__ __launch_bounds__(kTpThreads) void k_tp_coarse(
+    const int64_t *__restrict__ pcoff, const int32_t *__restrict__ pcrow,
+    const int32_t *__restrict__ padj, const int64_t *__restrict__ toff,
+    const int32_t *__restrict__ fb, const int64_t *__restrict__ ctl, int32_t nv, int64_t nplus,
+    unsigned long long *__restrict__ sc, unsigned long long *__restrict__ ccur,
+    uint64_t *__restrict__ rec_out, int32_t *__restrict__ own_out, int32_t *__restrict__ bad)
+{
+    __shared__ TpShared S;
+    __shared__ int32_t end_last;
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    const int64_t F = ctl[0], G = ctl[1];
+    const int32_t K = (int32_t)ctl[2];
+    const int32_t hub_lo = (int32_t)min(sc[1], (unsigned long long)nv);
+    const int64_t E0 = (int64_t)blockIdx.x * kTpTile, E1 = min(E0 + kTpTile, nplus);
+    int32_t *endm = S.own;  // row ends by tile entry (relative to E0)
+    for (int32_t b = t; b <= K; b += kTpThreads) S.bnd[b] = fb[min(G * b, F)];
+    for (int32_t b = t; b < K; b += kTpThreads) S.cnt[b] = 0;
+    for (int i = t; i < kTpTile; i += kTpThreads) endm[i] = INT32_MAX;
+    if (t == 0) end_last = INT32_MAX;
+    __syncthreads();
+    // rows that overlap the tile: from the row of entry E0 to the row of E1
+    const int32_t rlo = pcrow[E0 / kWave];
+    const int32_t rhi = E1 < nplus ? pcrow[E1 / kWave] : nv - 1;
+    for (int64_t r = rlo + t; r <= rhi; r += kTpThreads) {
+        const int64_t b = pcoff[r], e = pcoff[r + 1];
+        if (e > E0 && e <= E1) endm[e - 1 - E0] = (int32_t)(e - E0);
+        else if (e > E1 && b < E1) end_last = (int32_t)(e - E0);  // runs past the tile
+    }
+    __syncthreads();
+    {  // suffix minimum: thread t owns entries [8t, 8t + 8)
+        static_assert(kTpPer == 8, "two int4 per thread");
+        int4 *e4 = reinterpret_cast<int4 *>(endm);
+        const int4 a = e4[2 * t], c = e4[2 * t + 1];
+        int32_t v[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int k = 6; k >= 0; --k) v[k] = min(v[k], v[k + 1]);
+        int32_t x = v[0];  // minimum over the lanes from this one up
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int32_t y = __shfl_down(x, d, kWave);
+            if (lane + d < kWave) x = min(x, y);
+        }
+        if (lane == 0) S.wsum[wid] = x;
+        __syncthreads();
+        int32_t after = __shfl_down(x, 1, kWave);
+        if (lane == kWave - 1) after = INT32_MAX;
+        for (int q = wid + 1; q < kTpThreads / kWave; ++q) after = min(after, S.wsum[q]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = min(v[k], after);
+        e4[2 * t] = make_int4(v[0], v[1], v[2], v[3]);
+        e4[2 * t + 1] = make_int4(v[4], v[5], v[6], v[7]);
+        __syncthreads();
+    }
+    uint64_t rec[kTpPer];
+    int32_t own[kTpPer], bk[kTpPer], rank[kTpPer];
+    unsigned long long nhub = 0;
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k) {
+        const int64_t e = E0 + k * kTpThreads + t;
+        own[k] = e < E1 ? padj[e] : INT32_MAX;
+    }
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k) {
+        const int32_t i = k * kTpThreads + t;
+        const int64_t e = E0 + i;
+        bk[k] = -1;
+        rec[k] = 0;
+        rank[k] = 0;
+        if (e >= E1) continue;
+        if (own[k] >= hub_lo) {
+            ++nhub;  // filled by k_orient_transpose_hubs
+            continue;
+        }
+        bk[k] = 0;
+        int32_t end = endm[i];
+        if (end == INT32_MAX) end = end_last;
+        rec[k] = TaskRec::make(e + 1, end - i - 1).v;
+    }
+    tp_buckets(S.bnd, K, own, bk);
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k)
+        if (bk[k] >= 0) rank[k] = atomicAdd(&S.cnt[bk[k]], 1);
+    tp_split(S, K, rec, own, bk, rank, toff, ccur, rec_out, own_out, bad);
+    const unsigned long long part[1] = {nhub};
+    block_add_u64<1>(part, sc + 2);
+}
+
+// Pass 2: one tile of a coarse region into the fine buckets of that coarse
+// bucket (records into the task array, owners beside them).
+__global__ __launch_bounds__(kTpThreads) void k_tp_fine(
+    const uint64_t *__restrict__ rec_in, const int32_t *__restrict__ own_in,
+    const int64_t *__restrict__ toff, const int32_t *__restrict__ fb,
+    const int64_t *__restrict__ ctl, const int32_t *__restrict__ tiles,
+    unsigned long long *__restrict__ fcur, uint64_t *__restrict__ rec_out,
+    int32_t *__restrict__ own_out, int32_t *__restrict__ bad)
+{
+    __shared__ TpShared S;
+    __shared__ int32_t sk;
+    const int t = threadIdx.x;
+    const int64_t F = ctl[0], G = ctl[1], K = ctl[2], ntiles = ctl[3];
+    const int64_t bid = blockIdx.x;
+    if (bid >= ntiles) return;
+    if (t == 0) {  // the coarse bucket: last k with tiles[k] <= bid
+        int32_t lo = 0, hi = (int32_t)K - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (tiles[mid] <= bid) lo = mid; else hi = mid - 1;
+        }
+        sk = lo;
+    }
+    __syncthreads();
+    const int64_t k = sk;
+    const int64_t f0 = G * k;
+    const int32_t nf = (int32_t)min(G, F - f0);
+    for (int32_t b = t; b <= nf; b += kTpThreads) S.bnd[b] = fb[f0 + b];
+    for (int32_t b = t; b < nf; b += kTpThreads) S.cnt[b] = 0;
+    const int64_t r1 = toff[fb[f0 + nf]];
+    const int64_t p0 = toff[fb[f0]] + (bid - tiles[k]) * kTpTile;
+    const int64_t p1 = min(p0 + kTpTile, r1);
+    __syncthreads();
+    uint64_t rec[kTpPer];
+    int32_t own[kTpPer], bk[kTpPer], rank[kTpPer];
+#pragma unroll
+    for (int q = 0; q < kTpPer; ++q) {
+        const int64_t p = p0 + (int64_t)q * kTpThreads + t;
+        bk[q] = -1;
+        rec[q] = 0;
+        own[q] = 0;
+        rank[q] = 0;
+        if (p >= p1) continue;
+        own[q] = own_in[p];
+        rec[q] = rec_in[p];
+        bk[q] = 0;
+    }
+    tp_buckets(S.bnd, nf, own, bk);
+#pragma unroll
+    for (int q = 0; q < kTpPer; ++q)
+        if (bk[q] >= 0) rank[q] = atomicAdd(&S.cnt[bk[q]], 1);
+    tp_split(S, nf, rec, own, bk, rank, toff, fcur + f0, rec_out, own_out, bad);
+}
+
+// Pass 3: every fine bucket of several owners, counting-sorted by owner in
+// LDS: the record of owner w with rank s among w's records goes to
+// toff[w] + s.  Owner w's key toff[w] - (bucket start) is below kTpWin (all
+// owners of the bucket start their lists in one window), so counters by key
+// need no owner index.  A record of an owner outside the bucket, of an owner
+// without tasks, or past its owner's task count flags the rows.
+__global__ __launch_bounds__(kTpThreads) void k_tp_local(const int64_t *__restrict__ toff,
+                                                         const int32_t *__restrict__ fb,
+                                                         const int64_t *__restrict__ ctl,
+                                                         uint64_t *__restrict__ tasks,
+                                                         const int32_t *__restrict__ own,
+                                                         int32_t *__restrict__ bad)
+{
+    __shared__ int32_t lc[kTpWin];
+    __shared__ uint64_t st[kTpLocal];
+    const int t = threadIdx.x;
+    const int64_t F = ctl[0];
+    for (int64_t f = blockIdx.x; f < F; f += gridDim.x) {
+        const int32_t w0 = fb[f], w1 = fb[f + 1];
+        if (w1 - w0 <= 1) continue;  // one owner: its records are final
+        const int64_t rs = toff[w0], n = toff[w1] - rs;
+        if (n > kTpLocal) {  // not possible for buckets built by k_tp_flags
+            if (t == 0) atomicOr(bad, 1);
+            continue;
+        }
+        for (int i = t; i < kTpWin; i += kTpThreads) lc[i] = 0;
+        __syncthreads();
+        bool ok = true;
+        for (int64_t i = t; i < n; i += kTpThreads) {
+            const uint64_t r = tasks[rs + i];
+            const int32_t w = own[rs + i];
+            bool good = w >= w0 && w < w1;
+            if (good) {
+                const int64_t a = toff[w], d = toff[w + 1] - a, key = a - rs;
+                good = d > 0 && key >= 0 && key < kTpWin;
+                if (good) {
+                    const int32_t s = atomicAdd(&lc[key], 1);
+                    good = s < d;
+                    if (good) st[key + s] = r;
+                }
+            }
+            ok = ok && good;
+        }
+        if (!ok) atomicOr(bad, 1);
+        __syncthreads();
+        for (int64_t i = t; i < n; i += kTpThreads) tasks[rs + i] = st[i];
+        __syncthreads();
+    }
+}
+
+// Every coarse and fine region received exactly its owners' task count
+// (with the per-record checks of pass 3 this is the per-owner count check),
+// and the plus entries that point to hubs are as many as the hubs' tasks.
+__global__ void k_tp_check(const int64_t *__restrict__ toff, const int32_t *__restrict__ fb,
+                           const int64_t *__restrict__ ctl, const unsigned long long *__restrict__ ccur,
+                           const unsigned long long *__restrict__ fcur, int32_t nv,
+                           const unsigned long long *__restrict__ sc, int32_t *__restrict__ bad)
+{
+    const int64_t F = ctl[0], G = ctl[1], K = ctl[2];
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = gid; i < F; i += stride)
+        if ((int64_t)fcur[i] != toff[fb[i + 1]] - toff[fb[i]]) atomicOr(bad, 1);
+    for (int64_t k = gid; k < K; k += stride)
+        if ((int64_t)ccur[k] != toff[fb[min(G * (k + 1), F)]] - toff[fb[G * k]]) atomicOr(bad, 1);
+    if (gid == 0) {
+        const int64_t hub_lo = (int64_t)min(sc[1], (unsigned long long)nv);
+        if (sc[2] != (unsigned long long)(toff[nv] - toff[hub_lo])) atomicOr(bad, 1);
+    }
+}
+
 __global__ void k_direct_vinfo(const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
                                int32_t nv, VInfo *__restrict__ vi)
 {
@@ -1847,7 +2286,7 @@ namespace {
 constexpr int kL0 = 64, kL1 = 512, kL2 = 4096;
 // owners of at least this degree get their task lists from their own rows
 // (k_orient_transpose_hubs) instead of by appends
-constexpr int kHubDegree = 65536;
+constexpr int kHubDegree = 65536;  // with the appends
 constexpr int kNW0 = 1, kNW1 = 2, kNW2 = 8;
 
 // ev_long (bin 2, oriented): events recorded before and after the bitmap-tile
@@ -2006,8 +2445,6 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     TC_TRY(g.vinfo.ensure((size_t)nv * sizeof(VInfo)));
     VInfo *vi = g.vinfo.as<VInfo>();
     hipLaunchKernelGGL(k_rank_vinfo, dim3(vblocks), dim3(256), 0, s, pcoff, dpr, dmr, nv, vi);
-    hipLaunchKernelGGL(k_hub_lo, dim3(1), dim3(1), 0, s, vi, nv,
-                       hub_degree > 0 ? hub_degree : kHubDegree, sc);
     TC_HIP(hipGetLastError());
     int64_t tot[2] = {g.plan.nplus, g.plan.ntasks};
     if (!g.capturing) {
@@ -2103,17 +2540,69 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, toff, nv, g.crow.as<int32_t>());
     int32_t *cursor = dmo;  // store-order |N-| is not needed any more
     int32_t *bad = lcnt + 5;
-    hipLaunchKernelGGL(k_fill_u32, dim3(vblocks), dim3(256), 0, s, reinterpret_cast<uint32_t *>(cursor),
-                       (int64_t)nv, 0u);
-    if (nplus > 0) {
-        hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 8192)), dim3(256), 0, s,
-                           pcoff, pcrow, padj, toff, nv, nplus, g.tasks.as<TaskRec>(), cursor,
-                           sc, bad);
+    // the partitioned transpose's bound on its fine buckets (window starts,
+    // owners of more than a window and the owners after them) and its
+    // staging (16 bytes per task); the appends when either does not fit
+    const int64_t fmax = std::min<int64_t>(nv, 3 * (ntasks / kTpWin) + 1) + 1;
+    bool part = nplus > 0 && fmax <= (int64_t)kTpMaxB * kTpMaxB;
+    if (part) {
+        const size_t meta = (size_t)(4 + kTpMaxB + fmax) * 8 + (size_t)(fmax + 1 + kTpMaxB + 1) * 4;
+        part = g.ov[13].ensure((size_t)(nv + 1) * sizeof(int64_t)) == TC_OK &&
+               g.ov[14].ensure(meta) == TC_OK &&
+               g.ov[15].ensure((size_t)ntasks * sizeof(uint64_t)) == TC_OK &&
+               g.ov[16].ensure((size_t)ntasks * sizeof(int32_t)) == TC_OK &&
+               g.ov[17].ensure((size_t)ntasks * sizeof(int32_t)) == TC_OK;
+        if (!part)  // out of memory for the staging: free it again
+            for (int q = 13; q < 18; ++q) g.ov[q].release();
+    }
+    // hubs: owners whose task lists are built from their rows (the appends
+    // would serialise on their cursors; the partitioned transpose has no
+    // per-owner atomics and needs no hubs unless the schedule asks)
+    hipLaunchKernelGGL(k_hub_lo, dim3(1), dim3(1), 0, s, vi, nv,
+                       hub_degree > 0 ? hub_degree : part ? INT32_MAX : kHubDegree, sc);
+    TC_HIP(hipGetLastError());
+    if (part) {
+        int64_t *fpos = g.ov[13].as<int64_t>();
+        int64_t *ctl = g.ov[14].as<int64_t>();
+        unsigned long long *ccur = reinterpret_cast<unsigned long long *>(ctl + 4);
+        unsigned long long *fcur = ccur + kTpMaxB;
+        int32_t *fb = reinterpret_cast<int32_t *>(fcur + fmax);
+        int32_t *tiles = fb + fmax + 1;
+        uint64_t *rec1 = g.ov[15].as<uint64_t>();
+        int32_t *own1 = g.ov[16].as<int32_t>(), *own2 = g.ov[17].as<int32_t>();
+        uint64_t *tasks = reinterpret_cast<uint64_t *>(g.tasks.as<TaskRec>());
+        int32_t *flag = cursor;
+        hipLaunchKernelGGL(k_tp_flags, dim3(vblocks), dim3(256), 0, s, toff, sc, nv, flag);
+        TC_TRY(scan_counts_i32(flag, fpos, nv, g.tmp, s));
+        hipLaunchKernelGGL(k_tp_bounds, dim3(vblocks), dim3(256), 0, s, flag, fpos, sc, nv, fb);
+        hipLaunchKernelGGL(k_tp_plan, dim3(1), dim3(1024), 0, s, fpos, nv, fb, toff, ctl, tiles, ccur);
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(fmax, 256, 4096)), dim3(256), 0, s, fcur, fmax);
+        hipLaunchKernelGGL(k_tp_coarse, dim3((unsigned)ceil_div(nplus, kTpTile)), dim3(kTpThreads), 0, s,
+                           pcoff, pcrow, padj, toff, fb, ctl, nv, nplus, sc, ccur, rec1, own1, bad);
+        hipLaunchKernelGGL(k_tp_fine, dim3((unsigned)(ceil_div(ntasks, kTpTile) + kTpMaxB)),
+                           dim3(kTpThreads), 0, s, rec1, own1, toff, fb, ctl, tiles, fcur, tasks, own2,
+                           bad);
+        hipLaunchKernelGGL(k_tp_local, dim3((unsigned)std::min<int64_t>(fmax, 1 << 16)),
+                           dim3(kTpThreads), 0, s, toff, fb, ctl, tasks, own2, bad);
         hipLaunchKernelGGL(k_orient_transpose_hubs, dim3(cus * 16), dim3(256), 0, s, pcoff, toff,
                            order, rnk, ocoff, g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), mask,
                            cscan, padj, nv, sc, g.tasks.as<TaskRec>(), bad);
+        hipLaunchKernelGGL(k_tp_check, dim3(grid_for(fmax, 256, 4096)), dim3(256), 0, s, toff, fb, ctl,
+                           ccur, fcur, nv, sc, bad);
+    } else {
+        hipLaunchKernelGGL(k_fill_u32, dim3(vblocks), dim3(256), 0, s,
+                           reinterpret_cast<uint32_t *>(cursor), (int64_t)nv, 0u);
+        if (nplus > 0) {
+            hipLaunchKernelGGL(k_orient_transpose, dim3(grid_for(pchunks, 4, 8192)), dim3(256), 0, s,
+                               pcoff, pcrow, padj, toff, nv, nplus, g.tasks.as<TaskRec>(), cursor,
+                               sc, bad);
+            hipLaunchKernelGGL(k_orient_transpose_hubs, dim3(cus * 16), dim3(256), 0, s, pcoff, toff,
+                               order, rnk, ocoff, g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(),
+                               mask, cscan, padj, nv, sc, g.tasks.as<TaskRec>(), bad);
+        }
+        hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, toff, nv, sc,
+                           bad);
     }
-    hipLaunchKernelGGL(k_check_cursor, dim3(vblocks), dim3(256), 0, s, cursor, vi, toff, nv, sc, bad);
     TC_HIP(hipGetLastError());
     g.plan.bad = bad;
     int32_t hbad = 0;

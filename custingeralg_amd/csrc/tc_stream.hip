@@ -266,92 +266,150 @@ __global__ void k_unmark(const int32_t *__restrict__ queue, int64_t n0, const in
         mark[queue[i]] = 0;
 }
 
-// Removes flagged entries from each queued vertex's block (one wave per
-// vertex, order preserving, in place).
-__global__ void k_compact_rows(const int32_t *__restrict__ queue, int64_t n0,
-                               const int64_t *__restrict__ nd,
-                               const int64_t *__restrict__ boff, int32_t *__restrict__ len,
-                               int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
+// Entries per thread per pass of the row kernels: every pass issues all its
+// loads before it writes, so a long row costs its length / (threads * entries
+// per thread) round trips instead of one per thread-width chunk.
+constexpr int kRowPer = 8;        // one wavefront per row
+constexpr int kRowPerBlock = 16;  // one block per row: rows of kBigRow or more
+
+// Synchronisation of the threads that share a row: one wavefront (rows below
+// kBigRow) or one block of kRowThreads.  sync() orders every read before it
+// before every write after it; any() is sync() that also reports whether p
+// holds for some thread.
+template <int NT> struct RowGroup;
+template <> struct RowGroup<kWave> {
+    __device__ static int rank() { return threadIdx.x & (kWave - 1); }
+    __device__ static void sync()
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __device__ static bool any(bool p)
+    {
+        const bool r = __ballot(p) != 0ull;
+        sync();
+        return r;
+    }
+};
+template <> struct RowGroup<kRowThreads> {
+    __device__ static int rank() { return threadIdx.x; }
+    __device__ static void sync() { __syncthreads(); }
+    __device__ static bool any(bool p) { return __syncthreads_or(p) != 0; }
+};
+
+// Removes the flagged entries of one row in place, order preserving.  Each
+// pass reads NT * RP entries (entry r0 + k * NT + t to thread t), then,
+// after sync, writes every kept entry to its rank among the kept ones; kept
+// entries never move up, so a pass only writes where it or an earlier pass
+// has read.  The kept counts per (k, wave) are scanned in k-major order by
+// the first wave: cnt holds RP * NT / kWave words, pre one more.
+// Returns the new length.
+template <int NT, int RP>
+__device__ inline int32_t compact_row(int64_t base, int32_t l, int32_t *__restrict__ pool,
+                                      uint8_t *__restrict__ flag, int32_t *cnt, int32_t *pre)
 {
-    const int64_t n = bound(n0, nd);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    using G = RowGroup<NT>;
+    constexpr int NW = NT / kWave, E = RP * NW, PL = (E + kWave - 1) / kWave;
+    const int t = G::rank(), lane = threadIdx.x & (kWave - 1), wid = t / kWave;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; q < n;
-         q += wstride) {
-        const int32_t v = queue[q];
-        const int64_t base = boff[v];
-        const int32_t l = len[v];
-        if (l >= kBigRow) continue;  // k_compact_rows_block
-        int32_t w = 0;
-        for (int32_t r0 = 0; r0 < l; r0 += kWave) {
-            const int32_t r = r0 + lane;
-            int32_t x = 0;
+    int32_t w = 0;
+    for (int32_t r0 = 0; r0 < l; r0 += NT * RP) {
+        int32_t x[RP];
+        uint64_t m[RP];
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int32_t r = r0 + k * NT + t;
             bool keep = false;
+            x[k] = 0;
             if (r < l) {
-                x = pool[base + r];
+                x[k] = pool[base + r];
                 keep = flag[base + r] == 0;
             }
-            const uint64_t m = __ballot(keep);
-            __builtin_amdgcn_wave_barrier();
-            if (keep) {
-                const int32_t dst = w + __popcll(m & lt);
-                pool[base + dst] = x;
-                flag[base + dst] = 0;
-            }
-            w += __popcll(m);
+            m[k] = __ballot(keep);
+            if (lane == 0) cnt[k * NW + wid] = __popcll(m[k]);
         }
-        for (int32_t r = w + lane; r < l; r += kWave) flag[base + r] = 0;
+        G::sync();  // every read of the pass is done before any write
+        if (wid == 0) {
+            int32_t c[PL], sum = 0;
+#pragma unroll
+            for (int u = 0; u < PL; ++u) {
+                const int idx = lane * PL + u;
+                c[u] = idx < E ? cnt[idx] : 0;
+                sum += c[u];
+            }
+            int32_t incl = sum;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int32_t y = __shfl_up(incl, d, kWave);
+                if (lane >= d) incl += y;
+            }
+            int32_t run = incl - sum;
+#pragma unroll
+            for (int u = 0; u < PL; ++u) {
+                const int idx = lane * PL + u;
+                if (idx < E) pre[idx] = run;
+                run += c[u];
+            }
+            if (lane == kWave - 1) pre[E] = incl;
+        }
+        G::sync();
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int32_t r = r0 + k * NT + t;
+            if ((m[k] >> lane) & 1ull) {
+                const int32_t dst = w + pre[k * NW + wid] + __popcll(m[k] & lt);
+                if (dst != r) {  // an entry before the first removed one stays
+                    pool[base + dst] = x[k];
+                    flag[base + dst] = 0;
+                }
+            }
+        }
+        w += pre[E];
+    }
+    for (int32_t r = w + t; r < l; r += NT) flag[base + r] = 0;
+    return w;
+}
+
+// Removes flagged entries from each queued vertex's block (one wave per
+// vertex, order preserving, in place); rows of kBigRow entries or more are
+// left to k_compact_rows_block.
+__global__ __launch_bounds__(256) void k_compact_rows(const int32_t *__restrict__ queue, int64_t n0,
+                                                      const int64_t *__restrict__ nd,
+                                                      const int64_t *__restrict__ boff,
+                                                      int32_t *__restrict__ len,
+                                                      int32_t *__restrict__ pool,
+                                                      uint8_t *__restrict__ flag)
+{
+    __shared__ int32_t cnt[256 / kWave][kRowPer];
+    __shared__ int32_t pre[256 / kWave][kRowPer + 1];
+    const int64_t n = bound(n0, nd);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / kWave) + wid; q < n; q += wstride) {
+        const int32_t v = queue[q];
+        const int32_t l = len[v];
+        if (l >= kBigRow) continue;  // k_compact_rows_block
+        const int32_t w = compact_row<kWave, kRowPer>(boff[v], l, pool, flag, cnt[wid], pre[wid]);
         if (lane == 0) len[v] = w;
     }
 }
 
-// k_compact_rows for rows of at least kBigRow entries: one block per row,
-// chunks of blockDim entries, block-wide prefix of the kept entries.
+// k_compact_rows for rows of at least kBigRow entries: one block per row.
 __global__ __launch_bounds__(kRowThreads) void k_compact_rows_block(
     const int32_t *__restrict__ queue, int64_t n0, const int64_t *__restrict__ nd,
     const int64_t *__restrict__ boff,
     int32_t *__restrict__ len, int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
 {
+    __shared__ int32_t cnt[kRowPerBlock * (kRowThreads / kWave)];
+    __shared__ int32_t pre[kRowPerBlock * (kRowThreads / kWave) + 1];
     const int64_t n = bound(n0, nd);
-    __shared__ int32_t wsum[kRowThreads / kWave];
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const int nw = blockDim.x / kWave;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
         const int32_t v = queue[q];
         const int32_t l = len[v];
         if (l < kBigRow) continue;
-        const int64_t base = boff[v];
-        int32_t w = 0;
-        for (int32_t r0 = 0; r0 < l; r0 += blockDim.x) {
-            const int32_t r = r0 + threadIdx.x;
-            int32_t x = 0;
-            bool keep = false;
-            if (r < l) {
-                x = pool[base + r];
-                keep = flag[base + r] == 0;
-            }
-            const uint64_t m = __ballot(keep);
-            if (lane == 0) wsum[wid] = __popcll(m);
-            __syncthreads();  // every read of the chunk is done before any write
-            int32_t before = 0, total = 0;
-            for (int k = 0; k < nw; ++k) {
-                const int32_t c = wsum[k];
-                before += k < wid ? c : 0;
-                total += c;
-            }
-            if (keep) {
-                const int32_t dst = w + before + __popcll(m & lt);
-                pool[base + dst] = x;
-                flag[base + dst] = 0;
-            }
-            w += total;
-            __syncthreads();
-        }
-        for (int32_t r = w + threadIdx.x; r < l; r += blockDim.x) flag[base + r] = 0;
+        const int32_t w = compact_row<kRowThreads, kRowPerBlock>(boff[v], l, pool, flag, cnt, pre);
         if (threadIdx.x == 0) len[v] = w;
-        __syncthreads();
     }
 }
 
@@ -463,30 +521,111 @@ __global__ void k_need(const int32_t *__restrict__ sv, const int64_t *__restrict
     }
 }
 
-// Merges a vertex's sorted new entries into its block (one wave per vertex).
-// Phase 1 computes each new entry's final slot from the untouched old list;
-// phase 2 moves the old entries (into a fresh block, or in place from the
-// back in 64-wide chunks: a chunk's reads precede its writes and its
-// destinations never fall below the chunk); phase 3 writes the new entries
-// into the remaining slots with flag 1 and leaves their slots in newpos.
-__global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg,
-                             int64_t nseg0, const int64_t *__restrict__ nsegd, int64_t n0,
-                             const int64_t *__restrict__ nd,
-                             const uint64_t *__restrict__ keys, int bv,
-                             const int32_t *__restrict__ need, const int64_t *__restrict__ nscan,
-                             const unsigned long long *__restrict__ ctl, int64_t *__restrict__ boff,
-                             int32_t *__restrict__ len, int32_t *__restrict__ cap,
-                             int32_t *__restrict__ pool, uint8_t *__restrict__ flag,
-                             int64_t *__restrict__ newpos)
+// Merges a row's cn sorted new entries (keys[b0 .. b0+cn), low bv bits) into
+// its lo old entries: into the fresh block at db when the row moves, else in
+// place (db == ob).  Old entry j goes to db + j + A(j), A(j) = the new entries
+// below it; new entry i goes to db + i + a(i), a(i) = the old entries below
+// it, with flag 1, and its slot is left in newpos.  Passes of NT * RP old
+// entries run from the top down: a pass reads its entries (staged in ys) and
+// then writes only at or above its first entry, where it or an earlier pass
+// has read; it writes the new entries whose a(i) falls in it (with the top
+// pass, those above every old entry too).  In place, below a pass where no
+// old entry moved nothing moves and no new entry goes.  New entries are
+// searched in kl (LDS) when at most kcap of them, else in keys.  sh: 2 words.
+template <int NT, int RP>
+__device__ inline void merge_row(int64_t ob, int64_t db, int32_t lo, const uint64_t *__restrict__ keys,
+                                 int64_t b0, int32_t cn, uint64_t m, int32_t *__restrict__ pool,
+                                 uint8_t *__restrict__ flag, int64_t *__restrict__ newpos,
+                                 int32_t *ys, int32_t *kl, int32_t kcap, int32_t *sh)
 {
+    using G = RowGroup<NT>;
+    constexpr int S = NT * RP;
+    const int t = G::rank();
+    const bool lds = cn <= kcap;
+    if (lds) {
+        for (int32_t i = t; i < cn; i += NT) kl[i] = (int32_t)(keys[b0 + i] & m);
+        G::sync();
+    }
+    auto below = [&](int32_t y) {  // new entries below y
+        int32_t a = 0, b = cn;
+        while (a < b) {
+            const int32_t mid = (a + b) >> 1;
+            const int32_t x = lds ? kl[mid] : (int32_t)(keys[b0 + mid] & m);
+            if (x < y) a = mid + 1; else b = mid;
+        }
+        return a;
+    };
+    const bool inplace = db == ob;
+    for (int32_t j0 = ((lo - 1) / S) * S; j0 >= 0; j0 -= S) {
+        const int32_t j1 = min(j0 + S, lo);
+        int32_t y[RP], A[RP];
+        bool moved = false;
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int32_t j = j0 + k * NT + t;
+            y[k] = 0;
+            A[k] = 0;
+            if (j < j1) {
+                y[k] = pool[ob + j];
+                ys[j - j0] = y[k];
+                A[k] = below(y[k]);
+                moved = moved || A[k] > 0;
+            }
+        }
+        if (t == 0) {  // the new entries of this pass: [A(j0 - 1), A(j1 - 1)), up to cn at the top
+            sh[0] = j0 > 0 ? below(pool[ob + j0 - 1]) : 0;
+            sh[1] = j1 == lo ? cn : below(pool[ob + j1 - 1]);
+        }
+        const bool any = G::any(moved);  // every read of the pass is done before any write
+        const int32_t ilo = sh[0], ihi = sh[1];
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int32_t j = j0 + k * NT + t;
+            if (j < j1 && (A[k] > 0 || !inplace)) {
+                pool[db + j + A[k]] = y[k];
+                flag[db + j + A[k]] = 0;
+            }
+        }
+        for (int32_t i = ilo + t; i < ihi; i += NT) {
+            const int32_t x = lds ? kl[i] : (int32_t)(keys[b0 + i] & m);
+            int32_t a = 0, b = j1 - j0;  // a(i) - j0: staged entries below x
+            while (a < b) {
+                const int32_t mid = (a + b) >> 1;
+                if (ys[mid] < x) a = mid + 1; else b = mid;
+            }
+            const int64_t d = db + i + j0 + a;
+            pool[d] = x;
+            flag[d] = 1;
+            newpos[b0 + i] = d;
+        }
+        G::sync();  // ys and sh are rewritten by the next pass
+        if (inplace && !any) break;
+    }
+}
+
+// Merges each vertex's sorted new entries into its block (one wave per
+// vertex; rows of kBigRow entries or more are left to k_merge_rows_block).
+// Rows whose new block size need[s] is set move to the pool's reserved room
+// at pool_used + nscan[s].
+__global__ __launch_bounds__(256) void k_merge_rows(
+    const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg0,
+    const int64_t *__restrict__ nsegd, int64_t n0, const int64_t *__restrict__ nd,
+    const uint64_t *__restrict__ keys, int bv, const int32_t *__restrict__ need,
+    const int64_t *__restrict__ nscan, const unsigned long long *__restrict__ ctl,
+    int64_t *__restrict__ boff, int32_t *__restrict__ len, int32_t *__restrict__ cap,
+    int32_t *__restrict__ pool, uint8_t *__restrict__ flag, int64_t *__restrict__ newpos)
+{
+    constexpr int kKeys = 256;
+    __shared__ int32_t ys[256 / kWave][kWave * kRowPer];
+    __shared__ int32_t kl[256 / kWave][kKeys];
+    __shared__ int32_t sh[256 / kWave][2];
     if (!ctl[kCtlFits]) return;
     const int64_t nseg = bound(nseg0, nsegd), n = bound(n0, nd);
     const int64_t pool_used = (int64_t)ctl[kCtlBase];
-    const int lane = threadIdx.x & (kWave - 1);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t m = (1ull << bv) - 1;
-    for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; s < nseg;
-         s += wstride) {
+    for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / kWave) + wid; s < nseg; s += wstride) {
         const int32_t v = sv[s];
         const int32_t lo = len[v];
         if (lo >= kBigRow) continue;  // k_merge_rows_block
@@ -495,63 +634,15 @@ __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__re
         const int64_t ob = boff[v];
         const bool move = need[s] > 0;
         const int64_t db = move ? pool_used + nscan[s] : ob;
-        // phase 1: final slots of the new entries
-        for (int32_t i = lane; i < cn; i += kWave) {
-            const int32_t x = (int32_t)(keys[b0 + i] & m);
-            int32_t a = 0, b = lo;
-            while (a < b) {
-                int32_t mid = (a + b) >> 1;
-                if (pool[ob + mid] < x) a = mid + 1; else b = mid;
+        if (lo > 0) {
+            merge_row<kWave, kRowPer>(ob, db, lo, keys, b0, cn, m, pool, flag, newpos, ys[wid],
+                                      kl[wid], kKeys, sh[wid]);
+        } else {  // an empty row: the new entries in order
+            for (int32_t i = lane; i < cn; i += kWave) {
+                pool[db + i] = (int32_t)(keys[b0 + i] & m);
+                flag[db + i] = 1;
+                newpos[b0 + i] = db + i;
             }
-            newpos[b0 + i] = db + i + a;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // phase 2: old entries shift up by the number of new entries below them
-        if (move) {
-            for (int32_t j = lane; j < lo; j += kWave) {
-                const int32_t y = pool[ob + j];
-                int32_t a = 0, b = cn;
-                while (a < b) {
-                    int32_t mid = (a + b) >> 1;
-                    if ((int32_t)(keys[b0 + mid] & m) < y) a = mid + 1; else b = mid;
-                }
-                pool[db + j + a] = y;
-                flag[db + j + a] = 0;
-            }
-        } else {
-            const int32_t top = ((lo + kWave - 1) / kWave) * kWave;
-            for (int32_t j0 = top - kWave; j0 >= 0; j0 -= kWave) {
-                const int32_t j = j0 + lane;
-                int32_t y = 0, a = 0;
-                if (j < lo) {
-                    y = pool[ob + j];
-                    int32_t lo2 = 0, hi2 = cn;
-                    while (lo2 < hi2) {
-                        int32_t mid = (lo2 + hi2) >> 1;
-                        if ((int32_t)(keys[b0 + mid] & m) < y) lo2 = mid + 1; else hi2 = mid;
-                    }
-                    a = lo2;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                // shifts never decrease with j: below this chunk nothing moves
-                if (!__ballot(j < lo && a > 0)) break;
-                if (j < lo && a > 0) {
-                    pool[ob + j + a] = y;
-                    flag[ob + j + a] = 0;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // phase 3: new entries into the gaps
-        for (int32_t i = lane; i < cn; i += kWave) {
-            const int64_t d = newpos[b0 + i];
-            pool[d] = (int32_t)(keys[b0 + i] & m);
-            flag[d] = 1;
         }
         if (lane == 0) {
             len[v] = lo + cn;
@@ -563,8 +654,7 @@ __global__ void k_merge_rows(const int32_t *__restrict__ sv, const int64_t *__re
     }
 }
 
-// k_merge_rows for rows of at least kBigRow entries: the same three phases
-// with a whole block per row (block barriers between reads and writes).
+// k_merge_rows for rows of at least kBigRow entries: one block per row.
 __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
     const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg0,
     const int64_t *__restrict__ nsegd, int64_t n0, const int64_t *__restrict__ nd,
@@ -574,11 +664,14 @@ __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
     int32_t *__restrict__ len, int32_t *__restrict__ cap, int32_t *__restrict__ pool,
     uint8_t *__restrict__ flag, int64_t *__restrict__ newpos)
 {
+    constexpr int kKeys = 4096;
+    __shared__ int32_t ys[kRowThreads * kRowPerBlock];
+    __shared__ int32_t kl[kKeys];
+    __shared__ int32_t sh[2];
     if (!ctl[kCtlFits]) return;
     const int64_t nseg = bound(nseg0, nsegd), n = bound(n0, nd);
     const int64_t pool_used = (int64_t)ctl[kCtlBase];
     const uint64_t m = (1ull << bv) - 1;
-    const int32_t B = blockDim.x;
     for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
         const int32_t v = sv[s];
         const int32_t lo = len[v];
@@ -588,57 +681,8 @@ __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
         const int64_t ob = boff[v];
         const bool move = need[s] > 0;
         const int64_t db = move ? pool_used + nscan[s] : ob;
-        for (int32_t i = threadIdx.x; i < cn; i += B) {
-            const int32_t x = (int32_t)(keys[b0 + i] & m);
-            int32_t a = 0, b = lo;
-            while (a < b) {
-                const int32_t mid = (a + b) >> 1;
-                if (pool[ob + mid] < x) a = mid + 1; else b = mid;
-            }
-            newpos[b0 + i] = db + i + a;
-        }
-        __syncthreads();
-        if (move) {
-            for (int32_t j = threadIdx.x; j < lo; j += B) {
-                const int32_t y = pool[ob + j];
-                int32_t a = 0, b = cn;
-                while (a < b) {
-                    const int32_t mid = (a + b) >> 1;
-                    if ((int32_t)(keys[b0 + mid] & m) < y) a = mid + 1; else b = mid;
-                }
-                pool[db + j + a] = y;
-                flag[db + j + a] = 0;
-            }
-        } else {
-            const int32_t top = ((lo + B - 1) / B) * B;
-            for (int32_t j0 = top - B; j0 >= 0; j0 -= B) {
-                const int32_t j = j0 + threadIdx.x;
-                int32_t y = 0, a = 0;
-                if (j < lo) {
-                    y = pool[ob + j];
-                    int32_t lo2 = 0, hi2 = cn;
-                    while (lo2 < hi2) {
-                        const int32_t mid = (lo2 + hi2) >> 1;
-                        if ((int32_t)(keys[b0 + mid] & m) < y) lo2 = mid + 1; else hi2 = mid;
-                    }
-                    a = lo2;
-                }
-                // every read of the chunk is done before any write; below a
-                // chunk without shifts nothing moves
-                if (!__syncthreads_or(j < lo && a > 0)) break;
-                if (j < lo && a > 0) {
-                    pool[ob + j + a] = y;
-                    flag[ob + j + a] = 0;
-                }
-                __syncthreads();
-            }
-        }
-        __syncthreads();
-        for (int32_t i = threadIdx.x; i < cn; i += B) {
-            const int64_t d = newpos[b0 + i];
-            pool[d] = (int32_t)(keys[b0 + i] & m);
-            flag[d] = 1;
-        }
+        merge_row<kRowThreads, kRowPerBlock>(ob, db, lo, keys, b0, cn, m, pool, flag, newpos, ys, kl,
+                                             kKeys, sh);
         if (threadIdx.x == 0) {
             len[v] = lo + cn;
             if (move) {
@@ -646,7 +690,6 @@ __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
                 cap[v] = need[s];
             }
         }
-        __syncthreads();
     }
 }
 
@@ -1185,7 +1228,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
                            nseg, B2, ndir, dk, bv, need, nscan, ctl, g.st.boff.as<int64_t>(),
                            g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
                            g.flagpool.as<uint8_t>(), newpos);
-        hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(B2, 64, 1024)), dim3(kRowThreads), 0, st,
+        hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(B2, 2, 1024)), dim3(kRowThreads), 0, st,
                            sv, sbeg, B2, nseg, B2, ndir, dk, bv, need, nscan, ctl,
                            g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
                            g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>(), newpos);
@@ -1234,7 +1277,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, st, queue, B2,
                            qlen, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                            g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
-        hipLaunchKernelGGL(k_compact_rows_block, dim3(grid_for(B2, 64, 1024)), dim3(kRowThreads), 0, st,
+        hipLaunchKernelGGL(k_compact_rows_block, dim3(grid_for(B2, 2, 1024)), dim3(kRowThreads), 0, st,
                            queue, B2, qlen, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                            g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
         hipLaunchKernelGGL(k_unmark, dim3(gB2), dim3(256), 0, st, queue, B2, qlen,

@@ -139,7 +139,9 @@ typedef struct {
     int32_t flush_items;   /* min items per owner slot in one job (bounds the per-job */
                            /*   flush of third-vertex counters)                      */
     int32_t hub_degree;    /* oriented: owners of at least this degree build their   */
-                           /*   task lists from their own rows (0: default 65536)    */
+                           /*   task lists from their own rows (0: default, none;    */
+                           /*   65536 on graphs too large for the partitioned        */
+                           /*   transpose)                                            */
     int32_t mpath_min;     /* fixed heuristic: balanced pairs (long <= 2 * short)     */
                            /*   whose shorter list has at least this many entries    */
                            /*   run as a merge path (0, the default: never)          */
