@@ -25,7 +25,7 @@ namespace tc {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kBigRow = 4096;   // rows this long are compacted / merged by a whole block
+constexpr int kBigRow = 1024;   // rows this long are compacted / merged by a whole block
 constexpr int kRowThreads = 1024;
 constexpr int kDeltaBlocks = 4096;  // k_delta_lb's largest grid (4 waves per block)
 // The insertion's pool reservation, words of a device control block: the
@@ -314,20 +314,29 @@ __device__ inline int32_t compact_row(int64_t base, int32_t l, int32_t *__restri
     const int t = G::rank(), lane = threadIdx.x & (kWave - 1), wid = t / kWave;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     int32_t w = 0;
+    // the next pass's entries are loaded while this one is written: no pass
+    // writes at or past its own end
+    int32_t x[RP], xn[RP];
+    uint8_t f[RP], fn[RP];
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        const int32_t r = k * NT + t;
+        x[k] = r < l ? pool[base + r] : 0;
+        f[k] = r < l ? flag[base + r] : 1;
+    }
     for (int32_t r0 = 0; r0 < l; r0 += NT * RP) {
-        int32_t x[RP];
         uint64_t m[RP];
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             const int32_t r = r0 + k * NT + t;
-            bool keep = false;
-            x[k] = 0;
-            if (r < l) {
-                x[k] = pool[base + r];
-                keep = flag[base + r] == 0;
-            }
-            m[k] = __ballot(keep);
+            m[k] = __ballot(r < l && f[k] == 0);
             if (lane == 0) cnt[k * NW + wid] = __popcll(m[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int32_t r = r0 + NT * RP + k * NT + t;
+            xn[k] = r < l ? pool[base + r] : 0;
+            fn[k] = r < l ? flag[base + r] : 1;
         }
         G::sync();  // every read of the pass is done before any write
         if (wid == 0) {
@@ -366,6 +375,11 @@ __device__ inline int32_t compact_row(int64_t base, int32_t l, int32_t *__restri
             }
         }
         w += pre[E];
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            x[k] = xn[k];
+            f[k] = fn[k];
+        }
     }
     for (int32_t r = w + t; r < l; r += NT) flag[base + r] = 0;
     return w;
@@ -556,21 +570,33 @@ __device__ inline void merge_row(int64_t ob, int64_t db, int32_t lo, const uint6
         return a;
     };
     const bool inplace = db == ob;
-    for (int32_t j0 = ((lo - 1) / S) * S; j0 >= 0; j0 -= S) {
+    // the next (lower) pass's entries are loaded while this one is written:
+    // no pass writes below its own start
+    int32_t y[RP], yn[RP];
+    const int32_t jtop = ((lo - 1) / S) * S;
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        const int32_t j = jtop + k * NT + t;
+        y[k] = j < lo ? pool[ob + j] : 0;
+    }
+    for (int32_t j0 = jtop; j0 >= 0; j0 -= S) {
         const int32_t j1 = min(j0 + S, lo);
-        int32_t y[RP], A[RP];
+        int32_t A[RP];
         bool moved = false;
 #pragma unroll
         for (int k = 0; k < RP; ++k) {
             const int32_t j = j0 + k * NT + t;
-            y[k] = 0;
             A[k] = 0;
             if (j < j1) {
-                y[k] = pool[ob + j];
                 ys[j - j0] = y[k];
                 A[k] = below(y[k]);
                 moved = moved || A[k] > 0;
             }
+        }
+#pragma unroll
+        for (int k = 0; k < RP; ++k) {
+            const int32_t j = j0 - S + k * NT + t;  // a full pass when j0 > 0
+            yn[k] = j0 > 0 ? pool[ob + j] : 0;
         }
         if (t == 0) {  // the new entries of this pass: [A(j0 - 1), A(j1 - 1)), up to cn at the top
             sh[0] = j0 > 0 ? below(pool[ob + j0 - 1]) : 0;
@@ -600,6 +626,8 @@ __device__ inline void merge_row(int64_t ob, int64_t db, int32_t lo, const uint6
         }
         G::sync();  // ys and sh are rewritten by the next pass
         if (inplace && !any) break;
+#pragma unroll
+        for (int k = 0; k < RP; ++k) y[k] = yn[k];
     }
 }
 
@@ -1224,14 +1252,17 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     // the merges and the insertion's delta: run again after the pool grew
     auto merge_tail = [&](bool first, hipStream_t st) -> int {
         hipLaunchKernelGGL(k_pool_reserve, dim3(1), dim3(64), 0, st, nscan, B2, ctl);
-        hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, st, sv, sbeg, B2,
-                           nseg, B2, ndir, dk, bv, need, nscan, ctl, g.st.boff.as<int64_t>(),
-                           g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
-                           g.flagpool.as<uint8_t>(), newpos);
+        // the long rows first: a row merged by the wave kernel may grow past
+        // kBigRow, and the block kernel, which picks its rows by their length,
+        // would then merge it again
         hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(B2, 2, 1024)), dim3(kRowThreads), 0, st,
                            sv, sbeg, B2, nseg, B2, ndir, dk, bv, need, nscan, ctl,
                            g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
                            g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>(), newpos);
+        hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, st, sv, sbeg, B2,
+                           nseg, B2, ndir, dk, bv, need, nscan, ctl, g.st.boff.as<int64_t>(),
+                           g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
+                           g.flagpool.as<uint8_t>(), newpos);
         TC_HIP(hipGetLastError());
         if (count) {
             TC_HIP(rec(first ? 11 : 14, st));
@@ -1274,6 +1305,8 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         hipLaunchKernelGGL(k_queue_endpoints, dim3(gB), dim3(256), 0, st, da, db, pres, Bc, nD,
                            g.vmark.as<int32_t>(), queue, acc + 4);
         const int64_t *qlen = reinterpret_cast<const int64_t *>(acc + 4);
+        // (a row the wave kernel compacts only shrinks: the block kernel,
+        // which takes the rows of kBigRow entries or more, never sees it)
         hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, st, queue, B2,
                            qlen, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                            g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());

@@ -245,15 +245,17 @@ def _hub_graph(n, hubs, hub_deg, seed):
                              del_v=np.zeros(0, np.int32))
 
 
+@pytest.mark.parametrize("n,hub_deg", [(16384, 6000), (131072, 40000)])
 @pytest.mark.parametrize("slack", [0, 100])
-def test_hub_rows_merge_and_compact(slack):
-    """Rows of several thousand entries (the block-per-row merge and compaction):
-    inserts spread over a hub row, inserts that all sort after its entries, inserts
-    that all sort before them, and deletions from it; store and counts against the
-    oracle after every batch."""
-    n = 16384
-    off, adj = _hub_graph(n, 3, 6000, 5)
-    assert np.diff(off).max() >= 6000
+def test_hub_rows_merge_and_compact(slack, n, hub_deg):
+    """Rows of thousands of entries (the block-per-row merge and compaction;
+    40,000 entries take three passes of a block): inserts spread over a hub
+    row, inserts that all sort after its entries, inserts that all sort before
+    them, and deletions from it; with headroom (slack 100) the rows merge in
+    place, without it they move.  Store and counts against the oracle after
+    every batch."""
+    off, adj = _hub_graph(n, 3, hub_deg, 5)
+    assert np.diff(off).max() >= hub_deg
     rng = np.random.default_rng(9)
     g = T.cuStinger(off, adj, slack_pct=slack)
     delta = T.DeviceArray(n, np.uint32).zero()
@@ -300,6 +302,55 @@ def test_hub_rows_merge_and_compact(slack):
         now = O.custinger_count(cur_off, cur_adj)
         assert np.array_equal(base + delta.to_host(), now), f"batch {bi} ({plan}): per-vertex"
         assert total == int(now.astype(np.uint64).sum()), f"batch {bi} ({plan}): global"
+    g.close()
+
+
+@pytest.mark.parametrize("slack", [0, 100])
+def test_rows_grow_past_block_threshold(slack):
+    """Rows just below the length from which a whole block merges them (1024)
+    receive enough new entries to end above it, and a row above it loses
+    entries to end below it: every row is merged or compacted exactly once."""
+    n = 8192
+    rng = np.random.default_rng(12)
+    hu = np.repeat(np.arange(4, dtype=np.int32), 900)
+    hv = np.concatenate([rng.choice(np.arange(4, n, dtype=np.int32), 900, replace=False)
+                         for _ in range(4)])
+    bu, bv = rng.integers(4, n, size=(2, 30000)).astype(np.int32)  # background, no hubs
+    off, adj = O.apply_batch_csr(n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32),
+                                 ins_u=np.concatenate([hu, bu]), ins_v=np.concatenate([hv, bv]),
+                                 del_u=np.zeros(0, np.int32), del_v=np.zeros(0, np.int32))
+    assert np.diff(off)[:4].tolist() == [900] * 4
+    g = T.cuStinger(off, adj, slack_pct=slack)
+    delta = T.DeviceArray(n, np.uint32).zero()
+    base = O.custinger_count(off, adj)
+    total = int(base.astype(np.uint64).sum())
+    cur_off, cur_adj = off, adj
+    for bi in range(3):
+        us, vs, ds = [], [], []
+        for hub in range(4):
+            row = cur_adj[cur_off[hub]:cur_off[hub + 1]]
+            if bi == 2:  # shrink the grown rows back below the threshold
+                v = rng.choice(row, 300, replace=False).astype(np.int32)
+                d = np.ones(len(v), np.uint8)
+            else:
+                cand = np.setdiff1d(np.arange(4, n, dtype=np.int32), row)
+                v = rng.choice(cand, 200 if bi == 0 else 100, replace=False).astype(np.int32)
+                d = np.zeros(len(v), np.uint8)
+            us.append(np.full(len(v), hub, np.int32))
+            vs.append(v)
+            ds.append(d)
+        u, v, dl = np.concatenate(us), np.concatenate(vs), np.concatenate(ds)
+        dg, _ = T.batch_apply(g, u, v, dl, delta)
+        total += dg
+        want_off, want_adj = _host_apply(n, cur_off, cur_adj, u, v, dl)
+        got_off, got_adj = g.to_csr()
+        assert np.array_equal(got_off, want_off), f"batch {bi}: offsets"
+        assert np.array_equal(got_adj, want_adj), f"batch {bi}: adjacency"
+        cur_off, cur_adj = want_off, want_adj
+        now = O.custinger_count(cur_off, cur_adj)
+        assert np.array_equal(base + delta.to_host(), now), f"batch {bi}: per-vertex"
+        assert total == int(now.astype(np.uint64).sum()), f"batch {bi}: global"
+    assert np.diff(cur_off)[:4].tolist() == [900] * 4
     g.close()
 
 
