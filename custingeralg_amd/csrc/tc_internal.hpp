@@ -256,4 +256,56 @@ __host__ __device__ inline uint32_t lds_hash(int32_t w, int bits)
     return ((uint32_t)w * 0x9E3779B1u) >> (32 - bits);
 }
 
+// ---------------------------------------- one-block register bitonic --
+// A block of 1024 threads sorts kRegSort keys ascending: thread (wave w,
+// lane l) holds positions 128w + l (v0) and 128w + 64 + l (v1).  Partners
+// closer than 64 are exchanged by shuffles, those 64 apart are a lane's own
+// pair, and only the 10 steps with partners 128 or more apart go through LDS
+// (buf, double-buffered: one barrier each).  Every thread calls it.
+constexpr int kRegSort = 2048;
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t x, int j)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, j, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), j, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The value position i keeps at step (k, j) of the network, its partner p.
+__device__ inline uint64_t bitonic_keep(int i, int j, int k, uint64_t v, uint64_t p)
+{
+    const bool asc = (i & k) == 0, lower = (i & j) == 0;
+    return lower == asc ? (v < p ? v : p) : (v < p ? p : v);
+}
+
+__device__ inline void block_bitonic_2048(uint64_t &v0, uint64_t &v1, uint64_t (*buf)[kRegSort])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = 128 * w + lane, i1 = i0 + 64;
+    int par = 0;
+    for (int k = 2; k <= kRegSort; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 128) {
+                buf[par][i0] = v0;
+                buf[par][i1] = v1;
+                __syncthreads();
+                const uint64_t p0 = buf[par][i0 ^ j], p1 = buf[par][i1 ^ j];
+                v0 = bitonic_keep(i0, j, k, v0, p0);
+                v1 = bitonic_keep(i1, j, k, v1, p1);
+                par ^= 1;
+            } else if (j == 64) {
+                const uint64_t lo = v0 < v1 ? v0 : v1, hi = v0 < v1 ? v1 : v0;
+                const bool asc = (i0 & k) == 0;
+                v0 = asc ? lo : hi;
+                v1 = asc ? hi : lo;
+            } else {
+                const uint64_t p0 = shfl_xor_u64(v0, j), p1 = shfl_xor_u64(v1, j);
+                v0 = bitonic_keep(i0, j, k, v0, p0);
+                v1 = bitonic_keep(i1, j, k, v1, p1);
+            }
+        }
+    }
+    __syncthreads();  // buf may be reused by the caller
+}
+
 }  // namespace tc

@@ -323,55 +323,14 @@ __global__ __launch_bounds__(1024) void k_sort_small(uint64_t *__restrict__ keys
 }
 
 // Keys-only sorts of at most kRegSort keys: the same bitonic network with the
-// keys in registers, two per lane (wave w holds positions 128w + lane and
-// 128w + 64 + lane).  Partners closer than 64 are exchanged by shuffles,
-// those 64 apart are the lane's own pair, and only the 10 steps with partners
-// 128 or more apart go through LDS (double-buffered: one barrier each).
-constexpr int kRegSort = 2048;
-
-__device__ inline uint64_t shfl_xor_u64(uint64_t x, int j)
-{
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, j, 64);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), j, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// The value position i keeps at step (k, j) of the network, its partner p.
-__device__ inline uint64_t bitonic_keep(int i, int j, int k, uint64_t v, uint64_t p)
-{
-    const bool asc = (i & k) == 0, lower = (i & j) == 0;
-    return lower == asc ? (v < p ? v : p) : (v < p ? p : v);
-}
-
+// keys in registers (block_bitonic_2048).
 __global__ __launch_bounds__(1024) void k_sort_reg(uint64_t *__restrict__ keys, int64_t n)
 {
     __shared__ uint64_t buf[2][kRegSort];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i0 = 128 * w + lane, i1 = i0 + 64;
     uint64_t v0 = i0 < n ? keys[i0] : ~0ull, v1 = i1 < n ? keys[i1] : ~0ull;
-    int par = 0;
-    for (int k = 2; k <= kRegSort; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 128) {
-                buf[par][i0] = v0;
-                buf[par][i1] = v1;
-                __syncthreads();
-                const uint64_t p0 = buf[par][i0 ^ j], p1 = buf[par][i1 ^ j];
-                v0 = bitonic_keep(i0, j, k, v0, p0);
-                v1 = bitonic_keep(i1, j, k, v1, p1);
-                par ^= 1;
-            } else if (j == 64) {
-                const uint64_t lo = v0 < v1 ? v0 : v1, hi = v0 < v1 ? v1 : v0;
-                const bool asc = (i0 & k) == 0;
-                v0 = asc ? lo : hi;
-                v1 = asc ? hi : lo;
-            } else {
-                const uint64_t p0 = shfl_xor_u64(v0, j), p1 = shfl_xor_u64(v1, j);
-                v0 = bitonic_keep(i0, j, k, v0, p0);
-                v1 = bitonic_keep(i1, j, k, v1, p1);
-            }
-        }
-    }
+    block_bitonic_2048(v0, v1, buf);
     if (i0 < n) keys[i0] = v0;
     if (i1 < n) keys[i1] = v1;
 }

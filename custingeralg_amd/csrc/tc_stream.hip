@@ -1056,6 +1056,262 @@ template <class T> T *buf(Graph &g, int i, int64_t n)
         if (!(p)) return TC_ERR_OOM;                                                   \
     } while (0)
 
+// ------------------------------------------------- small-batch kernels --
+// A batch of at most kSmallPairs pairs (2 * kSmallPairs directed entries)
+// does its bookkeeping in two one-block kernels instead of twenty launches
+// of sorts, scans and element-wise steps: the same results, written to the
+// same arrays, for the merges and deltas that follow.
+constexpr int kSmallPairs = kRegSort / 2;
+
+// Exclusive scan of x0 (position 128w + lane) and x1 (position 128w + 64 +
+// lane) over the kRegSort positions of a 1024-thread block; ws: 16 words.
+__device__ inline void scan_pos_2048(int32_t x0, int32_t x1, int32_t &e0, int32_t &e1,
+                                     int32_t &total, int32_t *ws)
+{
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int32_t c0 = x0, c1 = x1;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int32_t y0 = __shfl_up(c0, d, kWave), y1 = __shfl_up(c1, d, kWave);
+        if (lane >= d) {
+            c0 += y0;
+            c1 += y1;
+        }
+    }
+    const int32_t t0 = __shfl(c0, kWave - 1, kWave), t1 = __shfl(c1, kWave - 1, kWave);
+    if (lane == 0) ws[wid] = t0 + t1;
+    __syncthreads();
+    int32_t before = 0;
+    total = 0;
+    for (int q = 0; q < 1024 / kWave; ++q) {
+        before += q < wid ? ws[q] : 0;
+        total += ws[q];
+    }
+    e0 = before + c0 - x0;
+    e1 = before + t0 + c1 - x1;
+    __syncthreads();  // ws is reused
+}
+
+// Exclusive scan of x0 (position t) and x1 (position 1024 + t), int64.
+__device__ inline void scan_rows_2048(int64_t x0, int64_t x1, int64_t &e0, int64_t &e1,
+                                      int64_t &total, int64_t *ws)
+{
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int64_t c0 = x0, c1 = x1;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int64_t y0 = __shfl_up(c0, d, kWave), y1 = __shfl_up(c1, d, kWave);
+        if (lane >= d) {
+            c0 += y0;
+            c1 += y1;
+        }
+    }
+    if (lane == kWave - 1) {
+        ws[wid] = c0;
+        ws[16 + wid] = c1;
+    }
+    __syncthreads();
+    int64_t b0 = 0, b1 = 0, r0 = 0, r1 = 0;
+    for (int q = 0; q < 16; ++q) {
+        b0 += q < wid ? ws[q] : 0;
+        b1 += q < wid ? ws[16 + q] : 0;
+        r0 += ws[q];
+        r1 += ws[16 + q];
+    }
+    e0 = b0 + c0 - x0;
+    e1 = r0 + b1 + c1 - x1;
+    total = r0 + r1;
+    __syncthreads();
+}
+
+// k_batch_keys + sort + k_unique_split + both scans + k_compact_pairs for a
+// batch of Bc <= kSmallPairs staged pairs: the unique insertions (ia, ib)
+// and deletions (da, db) in key order, their counts *nI and *nD, and the
+// deletion requests added to ndel[0].
+__global__ __launch_bounds__(1024) void k_small_front(
+    const int32_t *__restrict__ u, const int32_t *__restrict__ v, const uint8_t *__restrict__ del,
+    int64_t Bc, const int64_t *__restrict__ nreal, uint64_t skip, int bv, int directed,
+    int32_t *__restrict__ ia, int32_t *__restrict__ ib, int32_t *__restrict__ da,
+    int32_t *__restrict__ db, int64_t *__restrict__ nI, int64_t *__restrict__ nD,
+    unsigned long long *__restrict__ ndel)
+{
+    __shared__ uint64_t buf[2][kRegSort];
+    __shared__ int32_t ws[16];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int i0 = 128 * wid + lane, i1 = i0 + 64;
+    const int64_t lim = bound(Bc, nreal);
+    unsigned long long nd = 0;
+    auto key = [&](int64_t i) -> uint64_t {
+        if (i >= Bc) return ~0ull;  // past the batch: sorts last
+        if (i >= lim) return skip;  // size-class padding
+        const uint64_t a = (uint32_t)(directed ? u[i] : min(u[i], v[i]));
+        const uint64_t b = (uint32_t)(directed ? v[i] : max(u[i], v[i]));
+        const bool d = del && del[i];
+        nd += d;
+        return (a << (bv + 1)) | (b << 1) | (uint64_t)(d ? 1 : 0);
+    };
+    uint64_t v0 = key(i0), v1 = key(i1);
+    const unsigned long long part[1] = {nd};
+    block_add_u64<1>(part, ndel);
+    block_bitonic_2048(v0, v1, buf);
+    buf[0][i0] = v0;
+    buf[0][i1] = v1;
+    __syncthreads();
+    const uint64_t p0 = i0 > 0 ? buf[0][i0 - 1] : ~0ull, p1 = buf[0][i1 - 1];
+    const bool f0 = v0 < skip && (i0 == 0 || p0 != v0), f1 = v1 < skip && p1 != v1;
+    const int32_t fi0 = f0 && !(v0 & 1), fd0 = f0 && (v0 & 1);
+    const int32_t fi1 = f1 && !(v1 & 1), fd1 = f1 && (v1 & 1);
+    int32_t pi0, pi1, ti, pd0, pd1, td;
+    scan_pos_2048(fi0, fi1, pi0, pi1, ti, ws);
+    scan_pos_2048(fd0, fd1, pd0, pd1, td, ws);
+    const uint64_t m = (1ull << bv) - 1;
+    if (fi0) {
+        ia[pi0] = (int32_t)(v0 >> (bv + 1));
+        ib[pi0] = (int32_t)((v0 >> 1) & m);
+    }
+    if (fi1) {
+        ia[pi1] = (int32_t)(v1 >> (bv + 1));
+        ib[pi1] = (int32_t)((v1 >> 1) & m);
+    }
+    if (fd0) {
+        da[pd0] = (int32_t)(v0 >> (bv + 1));
+        db[pd0] = (int32_t)((v0 >> 1) & m);
+    }
+    if (fd1) {
+        da[pd1] = (int32_t)(v1 >> (bv + 1));
+        db[pd1] = (int32_t)((v1 >> 1) & m);
+    }
+    if (threadIdx.x == 0) {
+        *nI = ti;
+        *nD = td;
+    }
+}
+
+// k_not + its scan + k_scale_i64 + k_fill_keys + k_directed_keys + sort +
+// k_segment_heads + its scan + k_segments + k_need + its scan + the count of
+// moved rows, for Bc <= kSmallPairs pairs: absent[] (pairs not in the store
+// among the first *nI), *ndir (their directed entries), the sorted directed
+// keys dk[0..2Bc) (padded with pad), the rows that receive entries (sv, sbeg,
+// *nseg), the block sizes of those that move (need, nscan[0..2Bc], nscan[2Bc]
+// their total) and *nmoved.
+__global__ __launch_bounds__(1024) void k_small_insert(
+    const int32_t *__restrict__ ia, const int32_t *__restrict__ ib,
+    const int32_t *__restrict__ pres, int64_t Bc, const int64_t *__restrict__ nI, int bv,
+    int directed, uint64_t pad, const int32_t *__restrict__ len, const int32_t *__restrict__ cap,
+    int32_t mul_x16, int32_t add, int32_t *__restrict__ absent, int64_t *__restrict__ ndir,
+    uint64_t *__restrict__ dk, int32_t *__restrict__ sv, int64_t *__restrict__ sbeg,
+    int64_t *__restrict__ nseg_out, int32_t *__restrict__ need, int64_t *__restrict__ nscan,
+    int64_t *__restrict__ nmoved)
+{
+    __shared__ uint64_t buf[2][kRegSort];
+    __shared__ int32_t segb[kRegSort + 1];  // segment starts
+    __shared__ int32_t segv[kRegSort];      // segment rows
+    __shared__ int32_t ws[16];
+    __shared__ int64_t ws64[32];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    const int i0 = 128 * wid + lane, i1 = i0 + 64;
+    const int64_t B2 = 2 * Bc;
+    // absent pairs (one per thread) and their slots
+    const int64_t lim = bound(Bc, nI);
+    const int32_t ab = t < lim && !pres[t];
+    if (t < Bc) absent[t] = ab;
+    int32_t sidx, nnew;
+    {  // exclusive scan by pair index t: positions t map to (wave, lane) as t
+        int32_t c = ab;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int32_t y = __shfl_up(c, d, kWave);
+            if (lane >= d) c += y;
+        }
+        if (lane == kWave - 1) ws[wid] = c;
+        __syncthreads();
+        int32_t before = 0;
+        nnew = 0;
+        for (int q = 0; q < 16; ++q) {
+            before += q < wid ? ws[q] : 0;
+            nnew += ws[q];
+        }
+        sidx = before + c - ab;
+        __syncthreads();
+    }
+    const int64_t nd = directed ? nnew : 2 * (int64_t)nnew;
+    if (t == 0) *ndir = nd;
+    buf[0][i0] = pad;
+    buf[0][i1] = pad;
+    __syncthreads();
+    if (ab) {
+        const uint64_t a = (uint32_t)ia[t], b = (uint32_t)ib[t];
+        if (directed) {
+            buf[0][sidx] = (a << bv) | b;
+        } else {
+            buf[0][2 * sidx] = (a << bv) | b;
+            buf[0][2 * sidx + 1] = (b << bv) | a;
+        }
+    }
+    __syncthreads();
+    uint64_t v0 = buf[0][i0], v1 = buf[0][i1];
+    __syncthreads();
+    block_bitonic_2048(v0, v1, buf);
+    if (i0 < B2) dk[i0] = v0;
+    if (i1 < B2) dk[i1] = v1;
+    buf[0][i0] = v0;
+    buf[0][i1] = v1;
+    __syncthreads();
+    const uint64_t p0 = i0 > 0 ? buf[0][i0 - 1] : 0, p1 = buf[0][i1 - 1];
+    const int32_t h0 = v0 < pad && (i0 == 0 || (v0 >> bv) != (p0 >> bv));
+    const int32_t h1 = v1 < pad && (v1 >> bv) != (p1 >> bv);
+    int32_t s0, s1, nseg;
+    scan_pos_2048(h0, h1, s0, s1, nseg, ws);
+    if (h0) {
+        segb[s0] = i0;
+        segv[s0] = (int32_t)(v0 >> bv);
+        sv[s0] = (int32_t)(v0 >> bv);
+        sbeg[s0] = i0;
+    }
+    if (h1) {
+        segb[s1] = i1;
+        segv[s1] = (int32_t)(v1 >> bv);
+        sv[s1] = (int32_t)(v1 >> bv);
+        sbeg[s1] = i1;
+    }
+    if (t == 0) {
+        segb[nseg] = (int32_t)nd;
+        *nseg_out = nseg;
+    }
+    __syncthreads();
+    // block sizes of the rows that outgrow their blocks (rows s = t, 1024 + t)
+    int64_t c[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int s = q * 1024 + t;
+        c[q] = 0;
+        if (s < nseg) {
+            const int32_t r = segv[s];
+            const int64_t nl = (int64_t)len[r] + (segb[s + 1] - segb[s]);
+            if (nl > cap[r]) {
+                if (mul_x16 > 0) {
+                    c[q] = nl * mul_x16 / 16 + add;
+                } else {
+                    c[q] = 1;
+                    while (c[q] < nl) c[q] <<= 1;
+                }
+                c[q] = min<int64_t>(max<int64_t>(c[q], nl), INT32_MAX);
+            }
+        }
+        if (s < B2) need[s] = (int32_t)c[q];
+    }
+    int64_t e0, e1, grow;
+    scan_rows_2048(c[0], c[1], e0, e1, grow, ws64);
+    if (t < B2) nscan[t] = e0;
+    if (1024 + t < B2) nscan[1024 + t] = e1;
+    int64_t mv0, mv1, moved;
+    scan_rows_2048(c[0] != 0, c[1] != 0, mv0, mv1, moved, ws64);
+    if (t == 0) {
+        nscan[B2] = grow;
+        *nmoved = moved;
+    }
+}
+
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1232,6 +1488,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     int64_t *ndeleted = reinterpret_cast<int64_t *>(acc + 25);
     int64_t *nmoved = reinterpret_cast<int64_t *>(acc + 26);
     const int gB = grid_for(Bc, 256, 65536), gB2 = grid_for(B2, 256, 65536);
+    const bool small = Bc <= kSmallPairs;  // one-block bookkeeping
 
     uint32_t *out = delta;
     if (count && !out) {
@@ -1281,14 +1538,19 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     // the whole sequence after validation, staging and the pool's control words
     auto body = [&](hipStream_t st) -> int {
         // acc[5]: deletion requests of the raw batch
-        hipLaunchKernelGGL(k_batch_keys, dim3(gB), dim3(256), 0, st, su, svv, sdel, Bc, nreal, skip, bv,
-                           directed ? 1 : 0, keys, acc + 5);
-        TC_TRY(radix_sort_u64(keys, nullptr, Bc, bits, g.sb[1], g.sb[2], g.tmp, st));
-        hipLaunchKernelGGL(k_unique_split, dim3(gB), dim3(256), 0, st, keys, Bc, skip, fi, fd);
-        TC_TRY(scan_counts_i32(fi, pi, Bc, g.tmp, st));
-        TC_TRY(scan_counts_i32(fd, pd, Bc, g.tmp, st));
-        hipLaunchKernelGGL(k_compact_pairs, dim3(gB), dim3(256), 0, st, keys, Bc, bv, fi, pi, fd, pd,
-                           ia, ib, da, db);
+        if (small) {
+            hipLaunchKernelGGL(k_small_front, dim3(1), dim3(1024), 0, st, su, svv, sdel, Bc, nreal, skip,
+                               bv, directed ? 1 : 0, ia, ib, da, db, pi + Bc, pd + Bc, acc + 5);
+        } else {
+            hipLaunchKernelGGL(k_batch_keys, dim3(gB), dim3(256), 0, st, su, svv, sdel, Bc, nreal, skip,
+                               bv, directed ? 1 : 0, keys, acc + 5);
+            TC_TRY(radix_sort_u64(keys, nullptr, Bc, bits, g.sb[1], g.sb[2], g.tmp, st));
+            hipLaunchKernelGGL(k_unique_split, dim3(gB), dim3(256), 0, st, keys, Bc, skip, fi, fd);
+            TC_TRY(scan_counts_i32(fi, pi, Bc, g.tmp, st));
+            TC_TRY(scan_counts_i32(fd, pd, Bc, g.tmp, st));
+            hipLaunchKernelGGL(k_compact_pairs, dim3(gB), dim3(256), 0, st, keys, Bc, bv, fi, pi, fd,
+                               pd, ia, ib, da, db);
+        }
         TC_HIP(hipGetLastError());
         // ------------------------------------------------------ deletions
         hipLaunchKernelGGL(k_find_pairs, dim3(gB), dim3(256), 0, st, da, db, Bc, nD,
@@ -1327,26 +1589,33 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         hipLaunchKernelGGL(k_find_pairs, dim3(gB), dim3(256), 0, st, ia, ib, Bc, nI,
                            g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                            pab, pba, pres, directed ? 1 : 0);
-        hipLaunchKernelGGL(k_not, dim3(gB), dim3(256), 0, st, pres, Bc, nI, absent);
-        TC_TRY(scan_counts_i32(absent, sidx, Bc, g.tmp, st));
-        hipLaunchKernelGGL(k_scale_i64, dim3(1), dim3(64), 0, st, nNew, mult, ndir);
-        // directed keys of the new entries, padded to 2Bc, sorted
-        hipLaunchKernelGGL(k_fill_keys, dim3(gB2), dim3(256), 0, st, dk, B2, pad);
-        hipLaunchKernelGGL(k_directed_keys, dim3(gB), dim3(256), 0, st, ia, ib, sidx, Bc, nI, absent,
-                           bv, directed ? 1 : 0, dk);
-        TC_TRY(radix_sort_u64(dk, nullptr, B2, 2 * bv + 1, g.sb[1], g.sb[2], g.tmp, st));
-        hipLaunchKernelGGL(k_segment_heads, dim3(gB2), dim3(256), 0, st, dk, B2, bv, pad, head);
-        TC_TRY(scan_counts_i32(head, hpos, B2, g.tmp, st));
-        hipLaunchKernelGGL(k_segments, dim3(gB2), dim3(256), 0, st, dk, head, hpos, B2, bv, B2,
-                           g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), sv, sbeg, need);
-        hipLaunchKernelGGL(k_need, dim3(gB2), dim3(256), 0, st, sv, sbeg, B2, nseg, B2, ndir,
-                           g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.grow_mul_x16,
-                           g.grow_add, need);
-        TC_TRY(scan_counts_i32(need, nscan, B2, g.tmp, st));
-        // rows that move = rows with a block size
-        hipLaunchKernelGGL(k_nonzero, dim3(gB2), dim3(256), 0, st, need, B2, moved);
-        TC_TRY(scan_counts_i32(moved, mscan, B2, g.tmp, st));
-        hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, st, mscan + B2, nmoved, (int64_t)1);
+        if (small) {
+            hipLaunchKernelGGL(k_small_insert, dim3(1), dim3(1024), 0, st, ia, ib, pres, Bc, nI, bv,
+                               directed ? 1 : 0, pad, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
+                               g.grow_mul_x16, g.grow_add, absent, ndir, dk, sv, sbeg, hpos + B2, need,
+                               nscan, nmoved);
+        } else {
+            hipLaunchKernelGGL(k_not, dim3(gB), dim3(256), 0, st, pres, Bc, nI, absent);
+            TC_TRY(scan_counts_i32(absent, sidx, Bc, g.tmp, st));
+            hipLaunchKernelGGL(k_scale_i64, dim3(1), dim3(64), 0, st, nNew, mult, ndir);
+            // directed keys of the new entries, padded to 2Bc, sorted
+            hipLaunchKernelGGL(k_fill_keys, dim3(gB2), dim3(256), 0, st, dk, B2, pad);
+            hipLaunchKernelGGL(k_directed_keys, dim3(gB), dim3(256), 0, st, ia, ib, sidx, Bc, nI,
+                               absent, bv, directed ? 1 : 0, dk);
+            TC_TRY(radix_sort_u64(dk, nullptr, B2, 2 * bv + 1, g.sb[1], g.sb[2], g.tmp, st));
+            hipLaunchKernelGGL(k_segment_heads, dim3(gB2), dim3(256), 0, st, dk, B2, bv, pad, head);
+            TC_TRY(scan_counts_i32(head, hpos, B2, g.tmp, st));
+            hipLaunchKernelGGL(k_segments, dim3(gB2), dim3(256), 0, st, dk, head, hpos, B2, bv, B2,
+                               g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), sv, sbeg, need);
+            hipLaunchKernelGGL(k_need, dim3(gB2), dim3(256), 0, st, sv, sbeg, B2, nseg, B2, ndir,
+                               g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.grow_mul_x16,
+                               g.grow_add, need);
+            TC_TRY(scan_counts_i32(need, nscan, B2, g.tmp, st));
+            // rows that move = rows with a block size
+            hipLaunchKernelGGL(k_nonzero, dim3(gB2), dim3(256), 0, st, need, B2, moved);
+            TC_TRY(scan_counts_i32(moved, mscan, B2, g.tmp, st));
+            hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, st, mscan + B2, nmoved, (int64_t)1);
+        }
         TC_HIP(hipGetLastError());
         return merge_tail(true, st);
     };
