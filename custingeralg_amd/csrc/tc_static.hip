@@ -524,39 +524,92 @@ __global__ void k_sort_plus_wave(const VInfo *__restrict__ vi, const int32_t *__
     }
 }
 
-// Bitonic sort in LDS of the N+ part of the listed rows (one block per row,
-// rows of at most CAP entries).
+// Sort of the N+ part of the listed rows (one block per row, rows of at most
+// CAP entries): each thread sorts a run of PER = CAP / NT entries in
+// registers, then runs are merged pairwise in LDS, each thread producing PER
+// outputs of its pair after a merge-path split (a binary search on the
+// diagonal).  About PER + log2(run) LDS reads per thread and level, against
+// PER * log2(CAP) per level of a bitonic network over the whole row.  Only
+// the power of two of runs that covers the row takes part.
+#ifndef TC_SORT512_BLOCKS
+#define TC_SORT512_BLOCKS 64  // blocks per CU of the 512-entry row sort
+#endif
+#ifndef TC_SORT4096_BLOCKS
+#define TC_SORT4096_BLOCKS 8  // blocks per CU of the 4096-entry row sort
+#endif
 template <int CAP, int NT>
 __global__ __launch_bounds__(NT) void k_sort_plus_block(const VInfo *__restrict__ vi,
                                                         const int32_t *__restrict__ rows,
                                                         const int32_t *__restrict__ nrows,
                                                         int32_t *__restrict__ padj)
 {
+    constexpr int PER = CAP / NT;
+    static_assert(PER * NT == CAP && (PER & (PER - 1)) == 0, "runs of a power of two");
     __shared__ int32_t S[CAP];
+    const int t = threadIdx.x;
     const int32_t n = *nrows;
     for (int32_t q = blockIdx.x; q < n; q += gridDim.x) {
         const VInfo x = vi[rows[q]];
         const int32_t dp = x.len - x.dm;
         int32_t *P = padj + x.rb;
-        int32_t n2 = 1;
-        while (n2 < dp) n2 <<= 1;
-        for (int32_t i = threadIdx.x; i < n2; i += NT) S[i] = i < dp ? P[i] : INT32_MAX;
+        const int32_t runs = (dp + PER - 1) / PER;
+        int32_t m2 = 1;
+        while (m2 < runs) m2 <<= 1;
+        const int32_t m = m2 * PER;
+        const bool act = t < m2;
+        for (int32_t i = t; i < m; i += NT) S[i] = i < dp ? P[i] : INT32_MAX;
         __syncthreads();
-        for (int32_t k = 2; k <= n2; k <<= 1) {
-            for (int32_t j = k >> 1; j > 0; j >>= 1) {
-                for (int32_t i = threadIdx.x; i < (n2 >> 1); i += NT) {
-                    const int32_t a = 2 * i - (i & (j - 1));  // pair (a, a+j): bit j of a is 0
-                    const int32_t b = a + j;
-                    const int32_t x0 = S[a], x1 = S[b];
-                    if ((x0 > x1) == ((a & k) == 0)) {
-                        S[a] = x1;
-                        S[b] = x0;
+        int32_t r[PER];
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) r[k] = S[t * PER + k];
+#pragma unroll
+            for (int k = 2; k <= PER; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                    for (int i = 0; i < PER; ++i) {
+                        const int l = i ^ j;
+                        if (l > i) {
+                            const int32_t lo = min(r[i], r[l]), hi = max(r[i], r[l]);
+                            const bool up = (i & k) == 0;
+                            r[i] = up ? lo : hi;
+                            r[l] = up ? hi : lo;
+                        }
                     }
                 }
-                __syncthreads();
             }
+#pragma unroll
+            for (int k = 0; k < PER; ++k) S[t * PER + k] = r[k];  // its own run only
         }
-        for (int32_t i = threadIdx.x; i < dp; i += NT) P[i] = S[i];
+        __syncthreads();
+        for (int32_t w = PER; w < m; w <<= 1) {
+            if (act) {
+                const int32_t d0 = t * PER, base = d0 & ~(2 * w - 1), d = d0 - base;
+                const int32_t *A = S + base, *B = S + base + w;
+                int32_t lo = max(0, d - w), hi = min(d, w);
+                while (lo < hi) {  // entries taken from A among the first d
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (A[mid] <= B[d - 1 - mid]) lo = mid + 1; else hi = mid;
+                }
+                int32_t i = lo, j = d - lo;
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int32_t av = i < w ? A[i] : INT32_MAX, bv = j < w ? B[j] : INT32_MAX;
+                    const bool ta = j >= w || (i < w && av <= bv);
+                    r[k] = ta ? av : bv;
+                    i += ta ? 1 : 0;
+                    j += ta ? 0 : 1;
+                }
+            }
+            __syncthreads();  // every read of this level is done
+            if (act) {
+#pragma unroll
+                for (int k = 0; k < PER; ++k) S[t * PER + k] = r[k];
+            }
+            __syncthreads();
+        }
+        for (int32_t i = t; i < dp; i += NT) P[i] = S[i];
         __syncthreads();
     }
 }
@@ -1629,7 +1682,10 @@ __device__ inline void tile_filter(const int32_t *__restrict__ adj, int64_t &pb,
 // 4096 batched graphs, this beats 4 waves (123 VGPRs, no spills) and 8 waves
 // (64 VGPRs, about 60 spills).
 template <int LMAX, int NW, bool ORIENTED>
-__global__ __launch_bounds__(NW * kWave, LMAX <= 512 ? 6 : 1) void k_count(const Job *__restrict__ jobs, int64_t njobs,
+#ifndef TC_MID_WAVES
+#define TC_MID_WAVES 6
+#endif
+__global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_WAVES : 1) void k_count(const Job *__restrict__ jobs, int64_t njobs,
                                                       CountArgs A)
 {
     // The owner tile lives in a bucketed two-choice hash table: 4 ids per
@@ -2493,10 +2549,10 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     hipLaunchKernelGGL(k_sort_plus_wave, dim3(cus * 8), dim3(256), 0, s, vi, lst + hoff[0], lcnt + 0,
                        padj);
-    hipLaunchKernelGGL((k_sort_plus_block<512, 64>), dim3(cus * 16), dim3(64), 0, s, vi,
-                       lst + hoff[1], lcnt + 1, padj);
-    hipLaunchKernelGGL((k_sort_plus_block<4096, 256>), dim3(cus * 4), dim3(256), 0, s, vi,
-                       lst + hoff[2], lcnt + 2, padj);
+    hipLaunchKernelGGL((k_sort_plus_block<512, 64>), dim3(cus * TC_SORT512_BLOCKS), dim3(64), 0, s,
+                       vi, lst + hoff[1], lcnt + 1, padj);
+    hipLaunchKernelGGL((k_sort_plus_block<4096, 256>), dim3(cus * TC_SORT4096_BLOCKS), dim3(256), 0, s,
+                       vi, lst + hoff[2], lcnt + 2, padj);
     hipLaunchKernelGGL((k_sort_plus_block<16384, 1024>), dim3(cus), dim3(1024), 0, s, vi,
                        lst + hoff[3], lcnt + 3, padj);
     TC_HIP(hipGetLastError());
