@@ -137,7 +137,7 @@ struct Graph {
     DevBuf order, rnk, tasks;
     DevBuf hjobs;  // the largest bin's jobs left to its hash-table launch
     DevBuf outr;   // oriented per-vertex counts by rank, moved to vertex order after the count
-    DevBuf ov[18];
+    DevBuf ov[19];
     int64_t ntasks = 0;  // entries of the task array (oriented layout)
     int64_t nplus = 0;   // entries of the plus array (oriented layout)
     DevBuf pcnt;         // per N+ entry: hits of the task it became (summed per vertex)

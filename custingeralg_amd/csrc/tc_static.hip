@@ -381,6 +381,14 @@ __global__ void k_rank_keys_graph(const int64_t *__restrict__ voff, int32_t G, i
     }
 }
 
+// rnk[order[r]] = r.
+__global__ void k_rank_inverse(const int32_t *__restrict__ order, int32_t nv, int32_t *__restrict__ rnk)
+{
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
+         r += (int64_t)gridDim.x * blockDim.x)
+        rnk[order[r]] = (int32_t)r;
+}
+
 // rnk[order[r]] = r, and |N+| / |N-| of the vertex of rank r.
 __global__ void k_rank_of(const int32_t *__restrict__ order, const int32_t *__restrict__ len,
                           const int32_t *__restrict__ dmo, int32_t nv, int32_t *__restrict__ rnk,
@@ -434,6 +442,65 @@ __global__ void k_orient_write_plus(const int64_t *__restrict__ coff, const int3
         const int64_t rp = pe - plus_prefix(cscan, mask, ca);
         const int32_t b = pool[boff[a] + (e - ca)];
         padj[pcoff[rnk[a]] + rp] = rnk[b];
+    }
+}
+
+// With the ranks known first: the orientation mask by rank (rank(b) >
+// rank(a) is (degree, id) order, or (graph, degree, id) for batched graphs,
+// whose edges stay inside one graph) and every entry's neighbour as a rank,
+// tr[e], so that the N+ write reads those in order instead of looking up the
+// rank of each neighbour again.
+__global__ void k_orient_mask_rank(const int64_t *__restrict__ coff, const int32_t *__restrict__ crow,
+                                   const int64_t *__restrict__ boff, const int32_t *__restrict__ pool,
+                                   const int32_t *__restrict__ rnk, int32_t nv, int64_t nnz,
+                                   uint64_t *__restrict__ mask, int32_t *__restrict__ cnt,
+                                   int32_t *__restrict__ tr)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        const int64_t e = c * kWave + lane;
+        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
+        bool plus = false;
+        if (e < nnz) {
+            const int32_t rb = rnk[pool[boff[a] + (e - coff[a])]];
+            plus = rb > rnk[a];
+            tr[e] = rb;
+        }
+        const uint64_t m = __ballot(plus);
+        if (lane == 0) {
+            mask[c] = m;
+            cnt[c] = __popcll(m);
+        }
+    }
+}
+
+// k_orient_write_plus from the entries' ranks tr[] (k_orient_mask_rank).
+__global__ void k_orient_write_plus_rank(const int64_t *__restrict__ coff,
+                                         const int32_t *__restrict__ crow,
+                                         const int32_t *__restrict__ tr,
+                                         const int64_t *__restrict__ cscan,
+                                         const uint64_t *__restrict__ mask,
+                                         const int32_t *__restrict__ rnk,
+                                         const int64_t *__restrict__ pcoff, int32_t nv, int64_t nnz,
+                                         int32_t *__restrict__ padj)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += wstride) {
+        const int64_t e = c * kWave + lane;
+        const uint64_t m = mask[c];
+        if (!m) continue;  // (the whole wave)
+        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
+        if (e >= nnz || !((m >> lane) & 1ull)) continue;
+        const int64_t pe = cscan[c] + __popcll(m & lt);
+        const int64_t rp = pe - plus_prefix(cscan, mask, coff[a]);
+        padj[pcoff[rnk[a]] + rp] = tr[e];
     }
 }
 
@@ -2432,29 +2499,13 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
         return build_direct_vinfo(g);
     }
 
-    // ---- store-order pass: which entries point to a higher-rank vertex
+    // ---- store-order offsets and chunk rows
     TC_TRY(g.ov[0].ensure((size_t)(nv + 1) * sizeof(int64_t)));      // store-order offsets
     TC_TRY(g.ov[1].ensure((size_t)(nchunks + 1) * sizeof(int32_t))); // their chunk rows
     int64_t *ocoff = g.ov[0].as<int64_t>();
     int32_t *ocrow = g.ov[1].as<int32_t>();
     TC_TRY(scan_counts_i32(g.st.len.as<int32_t>(), ocoff, nv, g.tmp, s));
     hipLaunchKernelGGL(k_chunk_rows, dim3(vblocks), dim3(256), 0, s, ocoff, nv, ocrow);
-    // scratch: mask (u64 x nchunks), cscan (i64 x nchunks+1), cnt (i32 x nchunks)
-    TC_TRY(g.tmp2.ensure((size_t)nchunks * 8 + (size_t)(nchunks + 1) * 8 + (size_t)nchunks * 4 + 64));
-    uint64_t *mask = g.tmp2.as<uint64_t>();
-    int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
-    int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
-    hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
-                       g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(), nv,
-                       nnz, mask, cnt);
-    TC_HIP(hipGetLastError());
-    TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
-    TC_TRY(g.ov[2].ensure((size_t)nv * sizeof(int32_t)));  // |N-(v)| in store order
-    int32_t *dmo = g.ov[2].as<int32_t>();
-    hipLaunchKernelGGL(k_orient_dm, dim3(vblocks), dim3(256), 0, s, ocoff, g.st.len.as<int32_t>(),
-                       cscan, mask, nv, dmo);
-    TC_HIP(hipGetLastError());
-
     // ---- rank order: stable sort of the degrees
     TC_TRY(g.order.ensure((size_t)nv * sizeof(int32_t)));
     TC_TRY(g.rnk.ensure((size_t)nv * sizeof(int32_t)));
@@ -2485,6 +2536,36 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
         kbits += ceil_log2_i((int64_t)g.batch_ngraphs);
     }
     TC_TRY(radix_sort_u64(keys, order, nv, kbits, g.ov[4], g.ov[5], g.tmp, s));
+    hipLaunchKernelGGL(k_rank_inverse, dim3(vblocks), dim3(256), 0, s, order, nv, rnk);
+    TC_HIP(hipGetLastError());
+
+
+    // ---- store-order pass: which entries point to a higher-rank vertex,
+    // by rank, and every entry's neighbour as a rank for the N+ write.  If
+    // the 4 bytes per entry for those ranks cannot be allocated, the mask
+    // compares degrees and the N+ write looks the ranks up again.
+    // scratch: mask (u64 x nchunks), cscan (i64 x nchunks+1), cnt (i32 x nchunks)
+    TC_TRY(g.tmp2.ensure((size_t)nchunks * 8 + (size_t)(nchunks + 1) * 8 + (size_t)nchunks * 4 + 64));
+    uint64_t *mask = g.tmp2.as<uint64_t>();
+    int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
+    int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
+    const bool by_rank = g.ov[18].ensure((size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t)) == TC_OK;
+    int32_t *tr = by_rank ? g.ov[18].as<int32_t>() : nullptr;
+    if (by_rank)
+        hipLaunchKernelGGL(k_orient_mask_rank, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
+                           g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), rnk, nv, nnz, mask, cnt,
+                           tr);
+    else
+        hipLaunchKernelGGL(k_orient_mask, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
+                           g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.pool.as<int32_t>(),
+                           nv, nnz, mask, cnt);
+    TC_HIP(hipGetLastError());
+    TC_TRY(scan_counts_i32(cnt, cscan, nchunks, g.tmp, s));
+    TC_TRY(g.ov[2].ensure((size_t)nv * sizeof(int32_t)));  // |N-(v)| in store order
+    int32_t *dmo = g.ov[2].as<int32_t>();
+    hipLaunchKernelGGL(k_orient_dm, dim3(vblocks), dim3(256), 0, s, ocoff, g.st.len.as<int32_t>(),
+                       cscan, mask, nv, dmo);
+    TC_HIP(hipGetLastError());
     int32_t *dpr = reinterpret_cast<int32_t *>(keys);  // keys are no longer needed
     int32_t *dmr = dpr + nv;
     hipLaunchKernelGGL(k_rank_of, dim3(vblocks), dim3(256), 0, s, order, g.st.len.as<int32_t>(), dmo,
@@ -2514,9 +2595,13 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     TC_TRY(g.padj.ensure((size_t)std::max<int64_t>(nplus, 1) * sizeof(int32_t)));
     TC_TRY(g.tasks.ensure((size_t)std::max<int64_t>(ntasks, 1) * sizeof(TaskRec)));
     int32_t *padj = g.padj.as<int32_t>();
-    hipLaunchKernelGGL(k_orient_write_plus, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
-                       g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, rnk, pcoff, nv,
-                       nnz, padj);
+    if (by_rank)
+        hipLaunchKernelGGL(k_orient_write_plus_rank, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow, tr,
+                           cscan, mask, rnk, pcoff, nv, nnz, padj);
+    else
+        hipLaunchKernelGGL(k_orient_write_plus, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
+                           g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, rnk, pcoff,
+                           nv, nnz, padj);
     TC_HIP(hipGetLastError());
 
     // ---- sort every N+ part by rank
