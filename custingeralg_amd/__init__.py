@@ -66,7 +66,7 @@ class Schedule(C.Structure):
                 ("mid_max", C.c_int32), ("job_items", C.c_int32), ("ratio_x16", C.c_int32),
                 ("persistent", C.c_int32), ("flush_items", C.c_int32),
                 ("hub_degree", C.c_int32), ("mpath_min", C.c_int32), ("capture", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("build", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):

@@ -241,6 +241,7 @@ static int check_schedule(const tc_schedule *s)
                  "tc_schedule: bad policy");
     TC_CHECK_ARG(s->small_max >= 1 && s->mid_max >= s->small_max && s->job_items >= 1,
                  "tc_schedule: bad bin thresholds");
+    TC_CHECK_ARG(s->build == 0 || s->build == 1, "tc_schedule: bad build");
     return TC_OK;
 }
 

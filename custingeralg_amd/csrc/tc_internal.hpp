@@ -203,8 +203,9 @@ struct Graph {
 
 // Builds coff, crow and vinfo (and padj for the oriented layout) for the store;
 // hub_degree > 0 overrides the degree from which owners build their task
-// lists from their own rows.
-int prepare_views(Graph &g, bool oriented, int32_t hub_degree = 0);
+// lists from their own rows; fallback = 1 takes the paths used when the
+// default's staging does not fit (per-entry appends, mask by degree).
+int prepare_views(Graph &g, bool oriented, int32_t hub_degree = 0, int fallback = 0);
 // Fills vinfo with the store's rows as given (direct layout, dm=0).
 int build_direct_vinfo(Graph &g);
 // Validation of sortedness/range, self-loop count, optional symmetry check.

@@ -2478,7 +2478,7 @@ int build_direct_vinfo(Graph &g)
     return TC_OK;
 }
 
-int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
+int prepare_views(Graph &g, bool oriented, int32_t hub_degree, int fallback)
 {
     const int32_t nv = g.st.nv;
     const int64_t nnz = g.st.nnz;
@@ -2549,7 +2549,8 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     uint64_t *mask = g.tmp2.as<uint64_t>();
     int64_t *cscan = reinterpret_cast<int64_t *>(mask + nchunks);
     int32_t *cnt = reinterpret_cast<int32_t *>(cscan + nchunks + 1);
-    const bool by_rank = g.ov[18].ensure((size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t)) == TC_OK;
+    const bool by_rank =
+        !fallback && g.ov[18].ensure((size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t)) == TC_OK;
     int32_t *tr = by_rank ? g.ov[18].as<int32_t>() : nullptr;
     if (by_rank)
         hipLaunchKernelGGL(k_orient_mask_rank, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
@@ -2685,7 +2686,7 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree)
     // owners of more than a window and the owners after them) and its
     // staging (16 bytes per task); the appends when either does not fit
     const int64_t fmax = std::min<int64_t>(nv, 3 * (ntasks / kTpWin) + 1) + 1;
-    bool part = nplus > 0 && fmax <= (int64_t)kTpMaxB * kTpMaxB;
+    bool part = !fallback && nplus > 0 && fmax <= (int64_t)kTpMaxB * kTpMaxB;
     if (part) {
         const size_t meta = (size_t)(4 + kTpMaxB + fmax) * 8 + (size_t)(fmax + 1 + kTpMaxB + 1) * 4;
         part = g.ov[13].ensure((size_t)(nv + 1) * sizeof(int64_t)) == TC_OK &&
@@ -2848,7 +2849,7 @@ int count_pass(Graph &g, uint32_t *out, uint64_t *global, const tc_schedule &sch
     // compact offsets, chunk rows and the (degree-partitioned) vertex views;
     // rows declared symmetric that turn out not to be go the direct way
     if (oriented) {
-        const int pv = prepare_views(g, true, sched.hub_degree);
+        const int pv = prepare_views(g, true, sched.hub_degree, sched.build);
         if (pv == TC_ERR_INVALID && sched.method == TC_METHOD_AUTO && !g.symmetric) {
             oriented = false;
             method = TC_METHOD_DIRECT;

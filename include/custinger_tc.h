@@ -150,7 +150,11 @@ typedef struct {
                            /*   second such count records the whole sequence, later  */
                            /*   ones replay it (no host read-backs in between; the   */
                            /*   stats then give ms_total only, graph_replay = 1)     */
-    int32_t reserved[5];
+    int32_t build;         /* oriented layout: 0 (default) the partitioned transpose  */
+                           /*   and the mask by rank; 1 the paths the default falls  */
+                           /*   back to when their staging does not fit (per-entry   */
+                           /*   appends, mask by degree); the counts are the same    */
+    int32_t reserved[4];
 } tc_schedule;
 
 /* Fill *s with the defaults (autotuned table if one was loaded). */

@@ -51,7 +51,7 @@ def test_schedule_default_without_gpu():
     assert s.method == T.METHOD_AUTO and s.policy == T.POLICY_FIXED
     assert 1 <= s.small_max <= s.mid_max and s.job_items > 0
     # no merges by the fixed heuristic, built-in hub degree
-    assert s.ratio_x16 == 0 and s.hub_degree == 0
+    assert s.ratio_x16 == 0 and s.hub_degree == 0 and s.build == 0
 
 
 def test_host_validation_rejects_bad_csr():

@@ -45,6 +45,17 @@ def test_closed_forms_task_lists_from_rows(name):
     assert st["method_used"] == T.METHOD_ORIENTED
 
 
+@pytest.mark.parametrize("name", sorted(GR.closed_form_cases()))
+def test_closed_forms_fallback_build(name):
+    """The oriented layout's fallback paths (per-entry appends, mask by
+    degree) give the same counts as the default build."""
+    off, adj, t = GR.closed_form_cases()[name]
+    pv, glob, st = count(off, adj, build=1)
+    assert np.array_equal(pv, (2 * t).astype(np.uint32)), name
+    assert glob == 2 * int(t.sum())
+    assert st["method_used"] == T.METHOD_ORIENTED
+
+
 @pytest.mark.parametrize("name", sorted(GR.fixture_cases()))
 @pytest.mark.parametrize("method", [T.METHOD_AUTO, T.METHOD_DIRECT])
 def test_fixtures_networkx(name, method):
@@ -245,22 +256,25 @@ def _hub_graph(n, hubs, hub_deg, seed):
 
 @pytest.mark.parametrize("hub_degree", [0, 1, 3, 16384])
 @pytest.mark.parametrize("policy", POLICIES)
-def test_hub_owners(policy, hub_degree):
-    """Task lists built by appends and from the owners' own rows (every owner
+@pytest.mark.parametrize("build", [0, 1])
+def test_hub_owners(policy, hub_degree, build):
+    """Task lists built by the partitioned transpose (build 0) or by per-entry
+    appends (build 1, the fallback), and from the owners' own rows (every owner
     from degree 1, from degree 3, only the four hubs of degree 20000+, none at
     the default) give the same bit-exact counts."""
     off, adj = _hub_graph(40000, 4, 20000, 3)
     assert np.sort(np.diff(off))[-4] >= 16384
     want = O.custinger_count(off, adj)
-    pv, glob, st = count(off, adj, policy=policy, hub_degree=hub_degree)
+    pv, glob, st = count(off, adj, policy=policy, hub_degree=hub_degree, build=build)
     assert st["method_used"] == T.METHOD_ORIENTED
     assert np.array_equal(pv, want)
     assert glob == int(want.astype(np.uint64).sum())
 
 
+@pytest.mark.parametrize("build", [0, 1])
 @pytest.mark.parametrize("hub_degree", [0, 1, 16384])
 @pytest.mark.parametrize("drop", ["hub_side", "leaf_side"])
-def test_declared_symmetric_hub_rows_that_are_not(drop, hub_degree):
+def test_declared_symmetric_hub_rows_that_are_not(drop, hub_degree, build):
     """Rows declared symmetric (check skipped) with one entry of a hub pair
     missing: the task construction notices (appends or hub rows) and AUTO
     takes the direct method, whose result is the reference formula on the
@@ -281,12 +295,13 @@ def test_declared_symmetric_hub_rows_that_are_not(drop, hub_degree):
     want = O.custinger_count(off2, adj2)
     g = T.cuStinger(off2, adj2, assume_symmetric=True)
     try:
-        pv, glob, st = T.static_count(g, T.default_schedule(hub_degree=hub_degree))
+        pv, glob, st = T.static_count(g, T.default_schedule(hub_degree=hub_degree, build=build))
         assert st["method_used"] == T.METHOD_DIRECT
         assert np.array_equal(pv, want)
         assert glob == int(want.astype(np.uint64).sum())
         with pytest.raises(T.TCError):
-            T.static_count(g, T.default_schedule(method=T.METHOD_ORIENTED, hub_degree=hub_degree))
+            T.static_count(g, T.default_schedule(method=T.METHOD_ORIENTED, hub_degree=hub_degree,
+                                                 build=build))
     finally:
         g.close()
 
