@@ -459,17 +459,20 @@ __global__ void k_orient_mask_rank(const int64_t *__restrict__ coff, const int32
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nchunks = ceil_div(nnz, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         const int64_t e = c * kWave + lane;
         const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
         bool plus = false;
+        int32_t rb = 0;
         if (e < nnz) {
-            const int32_t rb = rnk[pool[boff[a] + (e - coff[a])]];
+            rb = rnk[pool[boff[a] + (e - coff[a])]];
             plus = rb > rnk[a];
-            tr[e] = rb;
         }
         const uint64_t m = __ballot(plus);
+        // the chunk's plus entries, compacted to the front of its 64 slots
+        if (plus) tr[c * kWave + __popcll(m & lt)] = rb;
         if (lane == 0) {
             mask[c] = m;
             cnt[c] = __popcll(m);
@@ -477,14 +480,25 @@ __global__ void k_orient_mask_rank(const int64_t *__restrict__ coff, const int32
     }
 }
 
-// k_orient_write_plus from the entries' ranks tr[] (k_orient_mask_rank).
+// Per store row a: where its plus entries go, less their store-order rank
+// among all plus entries (pdst[a] + plus index = slot in the plus array).
+__global__ void k_plus_dst(const int64_t *__restrict__ coff, const int64_t *__restrict__ cscan,
+                           const uint64_t *__restrict__ mask, const int32_t *__restrict__ rnk,
+                           const int64_t *__restrict__ pcoff, int32_t nv, int64_t *__restrict__ pdst)
+{
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < nv;
+         a += (int64_t)gridDim.x * blockDim.x)
+        pdst[a] = pcoff[rnk[a]] - plus_prefix(cscan, mask, coff[a]);
+}
+
+// k_orient_write_plus from the entries' ranks (k_orient_mask_rank, compacted
+// per chunk) and the rows' destinations (k_plus_dst).
 __global__ void k_orient_write_plus_rank(const int64_t *__restrict__ coff,
                                          const int32_t *__restrict__ crow,
                                          const int32_t *__restrict__ tr,
                                          const int64_t *__restrict__ cscan,
                                          const uint64_t *__restrict__ mask,
-                                         const int32_t *__restrict__ rnk,
-                                         const int64_t *__restrict__ pcoff, int32_t nv, int64_t nnz,
+                                         const int64_t *__restrict__ pdst, int32_t nv, int64_t nnz,
                                          int32_t *__restrict__ padj)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -498,9 +512,8 @@ __global__ void k_orient_write_plus_rank(const int64_t *__restrict__ coff,
         if (!m) continue;  // (the whole wave)
         const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
         if (e >= nnz || !((m >> lane) & 1ull)) continue;
-        const int64_t pe = cscan[c] + __popcll(m & lt);
-        const int64_t rp = pe - plus_prefix(cscan, mask, coff[a]);
-        padj[pcoff[rnk[a]] + rp] = tr[e];
+        const int32_t k = __popcll(m & lt);
+        padj[pdst[a] + cscan[c] + k] = tr[c * kWave + k];
     }
 }
 
@@ -2596,10 +2609,13 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree, int fallback)
     TC_TRY(g.padj.ensure((size_t)std::max<int64_t>(nplus, 1) * sizeof(int32_t)));
     TC_TRY(g.tasks.ensure((size_t)std::max<int64_t>(ntasks, 1) * sizeof(TaskRec)));
     int32_t *padj = g.padj.as<int32_t>();
-    if (by_rank)
+    if (by_rank) {
+        int64_t *pdst = reinterpret_cast<int64_t *>(keys);  // (nv words; dpr/dmr are read by now)
+        hipLaunchKernelGGL(k_plus_dst, dim3(vblocks), dim3(256), 0, s, ocoff, cscan, mask, rnk, pcoff,
+                           nv, pdst);
         hipLaunchKernelGGL(k_orient_write_plus_rank, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow, tr,
-                           cscan, mask, rnk, pcoff, nv, nnz, padj);
-    else
+                           cscan, mask, pdst, nv, nnz, padj);
+    } else
         hipLaunchKernelGGL(k_orient_write_plus, dim3(wblocks), dim3(256), 0, s, ocoff, ocrow,
                            g.st.boff.as<int64_t>(), g.st.pool.as<int32_t>(), cscan, mask, rnk, pcoff,
                            nv, nnz, padj);
