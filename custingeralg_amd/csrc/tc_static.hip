@@ -972,6 +972,16 @@ __global__ __launch_bounds__(1024) void k_tp_plan(const int64_t *__restrict__ fp
     }
 }
 
+// tmap[i] = the coarse bucket of pass-2 tile i (tiles[k] <= i < tiles[k+1]);
+// one block per coarse bucket.
+__global__ void k_tp_tilemap(const int64_t *__restrict__ ctl, const int32_t *__restrict__ tiles,
+                             int32_t *__restrict__ tmap)
+{
+    const int64_t k = blockIdx.x;
+    if (k >= ctl[2]) return;
+    for (int32_t i = tiles[k] + threadIdx.x; i < tiles[k + 1]; i += blockDim.x) tmap[i] = (int32_t)k;
+}
+
 struct TpShared {
     int32_t bnd[kTpMaxB + 1];  // owner bounds of the tile's buckets
     int32_t cnt[kTpMaxB];      // records per bucket, then their local offsets
@@ -1089,6 +1099,12 @@ __global__ __launch_bounds__(kTpThreads) void k_tp_coarse(
     const int32_t K = (int32_t)ctl[2];
     const int32_t hub_lo = (int32_t)min(sc[1], (unsigned long long)nv);
     const int64_t E0 = (int64_t)blockIdx.x * kTpTile, E1 = min(E0 + kTpTile, nplus);
+    int32_t own[kTpPer];  // the tile's entries, loaded first: in flight while the rows are marked
+#pragma unroll
+    for (int k = 0; k < kTpPer; ++k) {
+        const int64_t e = E0 + k * kTpThreads + t;
+        own[k] = e < E1 ? padj[e] : INT32_MAX;
+    }
     int32_t *endm = S.own;  // row ends by tile entry (relative to E0)
     for (int32_t b = t; b <= K; b += kTpThreads) S.bnd[b] = fb[min(G * b, F)];
     for (int32_t b = t; b < K; b += kTpThreads) S.cnt[b] = 0;
@@ -1129,13 +1145,8 @@ __global__ __launch_bounds__(kTpThreads) void k_tp_coarse(
         __syncthreads();
     }
     uint64_t rec[kTpPer];
-    int32_t own[kTpPer], bk[kTpPer], rank[kTpPer];
+    int32_t bk[kTpPer], rank[kTpPer];
     unsigned long long nhub = 0;
-#pragma unroll
-    for (int k = 0; k < kTpPer; ++k) {
-        const int64_t e = E0 + k * kTpThreads + t;
-        own[k] = e < E1 ? padj[e] : INT32_MAX;
-    }
 #pragma unroll
     for (int k = 0; k < kTpPer; ++k) {
         const int32_t i = k * kTpThreads + t;
@@ -1168,47 +1179,33 @@ __global__ __launch_bounds__(kTpThreads) void k_tp_fine(
     const uint64_t *__restrict__ rec_in, const int32_t *__restrict__ own_in,
     const int64_t *__restrict__ toff, const int32_t *__restrict__ fb,
     const int64_t *__restrict__ ctl, const int32_t *__restrict__ tiles,
-    unsigned long long *__restrict__ fcur, uint64_t *__restrict__ rec_out,
-    int32_t *__restrict__ own_out, int32_t *__restrict__ bad)
+    const int32_t *__restrict__ tmap, unsigned long long *__restrict__ fcur,
+    uint64_t *__restrict__ rec_out, int32_t *__restrict__ own_out, int32_t *__restrict__ bad)
 {
     __shared__ TpShared S;
-    __shared__ int32_t sk;
     const int t = threadIdx.x;
-    const int64_t F = ctl[0], G = ctl[1], K = ctl[2], ntiles = ctl[3];
+    const int64_t F = ctl[0], G = ctl[1], ntiles = ctl[3];
     const int64_t bid = blockIdx.x;
     if (bid >= ntiles) return;
-    if (t == 0) {  // the coarse bucket: last k with tiles[k] <= bid
-        int32_t lo = 0, hi = (int32_t)K - 1;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi + 1) >> 1;
-            if (tiles[mid] <= bid) lo = mid; else hi = mid - 1;
-        }
-        sk = lo;
-    }
-    __syncthreads();
-    const int64_t k = sk;
+    const int64_t k = tmap[bid];  // the coarse bucket
     const int64_t f0 = G * k;
     const int32_t nf = (int32_t)min(G, F - f0);
-    for (int32_t b = t; b <= nf; b += kTpThreads) S.bnd[b] = fb[f0 + b];
-    for (int32_t b = t; b < nf; b += kTpThreads) S.cnt[b] = 0;
     const int64_t r1 = toff[fb[f0 + nf]];
     const int64_t p0 = toff[fb[f0]] + (bid - tiles[k]) * kTpTile;
     const int64_t p1 = min(p0 + kTpTile, r1);
-    __syncthreads();
     uint64_t rec[kTpPer];
     int32_t own[kTpPer], bk[kTpPer], rank[kTpPer];
 #pragma unroll
-    for (int q = 0; q < kTpPer; ++q) {
+    for (int q = 0; q < kTpPer; ++q) {  // in flight while the bounds are set up
         const int64_t p = p0 + (int64_t)q * kTpThreads + t;
-        bk[q] = -1;
-        rec[q] = 0;
-        own[q] = 0;
+        bk[q] = p < p1 ? 0 : -1;
+        rec[q] = p < p1 ? rec_in[p] : 0;
+        own[q] = p < p1 ? own_in[p] : 0;
         rank[q] = 0;
-        if (p >= p1) continue;
-        own[q] = own_in[p];
-        rec[q] = rec_in[p];
-        bk[q] = 0;
     }
+    for (int32_t b = t; b <= nf; b += kTpThreads) S.bnd[b] = fb[f0 + b];
+    for (int32_t b = t; b < nf; b += kTpThreads) S.cnt[b] = 0;
+    __syncthreads();
     tp_buckets(S.bnd, nf, own, bk);
 #pragma unroll
     for (int q = 0; q < kTpPer; ++q)
@@ -2704,7 +2701,9 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree, int fallback)
     const int64_t fmax = std::min<int64_t>(nv, 3 * (ntasks / kTpWin) + 1) + 1;
     bool part = !fallback && nplus > 0 && fmax <= (int64_t)kTpMaxB * kTpMaxB;
     if (part) {
-        const size_t meta = (size_t)(4 + kTpMaxB + fmax) * 8 + (size_t)(fmax + 1 + kTpMaxB + 1) * 4;
+        const int64_t tmax = ceil_div(ntasks, kTpTile) + kTpMaxB;  // pass-2 tiles, at most
+        const size_t meta =
+            (size_t)(4 + kTpMaxB + fmax) * 8 + (size_t)(fmax + 1 + kTpMaxB + 1 + tmax) * 4;
         part = g.ov[13].ensure((size_t)(nv + 1) * sizeof(int64_t)) == TC_OK &&
                g.ov[14].ensure(meta) == TC_OK &&
                g.ov[15].ensure((size_t)ntasks * sizeof(uint64_t)) == TC_OK &&
@@ -2726,6 +2725,7 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree, int fallback)
         unsigned long long *fcur = ccur + kTpMaxB;
         int32_t *fb = reinterpret_cast<int32_t *>(fcur + fmax);
         int32_t *tiles = fb + fmax + 1;
+        int32_t *tmap = tiles + kTpMaxB + 1;
         uint64_t *rec1 = g.ov[15].as<uint64_t>();
         int32_t *own1 = g.ov[16].as<int32_t>(), *own2 = g.ov[17].as<int32_t>();
         uint64_t *tasks = reinterpret_cast<uint64_t *>(g.tasks.as<TaskRec>());
@@ -2737,9 +2737,10 @@ int prepare_views(Graph &g, bool oriented, int32_t hub_degree, int fallback)
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(fmax, 256, 4096)), dim3(256), 0, s, fcur, fmax);
         hipLaunchKernelGGL(k_tp_coarse, dim3((unsigned)ceil_div(nplus, kTpTile)), dim3(kTpThreads), 0, s,
                            pcoff, pcrow, padj, toff, fb, ctl, nv, nplus, sc, ccur, rec1, own1, bad);
+        hipLaunchKernelGGL(k_tp_tilemap, dim3(kTpMaxB), dim3(256), 0, s, ctl, tiles, tmap);
         hipLaunchKernelGGL(k_tp_fine, dim3((unsigned)(ceil_div(ntasks, kTpTile) + kTpMaxB)),
-                           dim3(kTpThreads), 0, s, rec1, own1, toff, fb, ctl, tiles, fcur, tasks, own2,
-                           bad);
+                           dim3(kTpThreads), 0, s, rec1, own1, toff, fb, ctl, tiles, tmap, fcur, tasks,
+                           own2, bad);
         hipLaunchKernelGGL(k_tp_local, dim3((unsigned)std::min<int64_t>(fmax, 1 << 16)),
                            dim3(kTpThreads), 0, s, toff, fb, ctl, tasks, own2, bad);
         hipLaunchKernelGGL(k_orient_transpose_hubs, dim3(cus * 16), dim3(256), 0, s, pcoff, toff,
