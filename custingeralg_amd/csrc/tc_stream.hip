@@ -1193,7 +1193,8 @@ __global__ __launch_bounds__(1024) void k_small_front(
 // among the first *nI), *ndir (their directed entries), the sorted directed
 // keys dk[0..2Bc) (padded with pad), the rows that receive entries (sv, sbeg,
 // *nseg), the block sizes of those that move (need, nscan[0..2Bc], nscan[2Bc]
-// their total) and *nmoved.
+// their total), *nmoved, and (ctl given) the pool reservation that
+// k_pool_reserve makes.
 __global__ __launch_bounds__(1024) void k_small_insert(
     const int32_t *__restrict__ ia, const int32_t *__restrict__ ib,
     const int32_t *__restrict__ pres, int64_t Bc, const int64_t *__restrict__ nI, int bv,
@@ -1201,7 +1202,7 @@ __global__ __launch_bounds__(1024) void k_small_insert(
     int32_t mul_x16, int32_t add, int32_t *__restrict__ absent, int64_t *__restrict__ ndir,
     uint64_t *__restrict__ dk, int32_t *__restrict__ sv, int64_t *__restrict__ sbeg,
     int64_t *__restrict__ nseg_out, int32_t *__restrict__ need, int64_t *__restrict__ nscan,
-    int64_t *__restrict__ nmoved)
+    int64_t *__restrict__ nmoved, unsigned long long *__restrict__ ctl)
 {
     __shared__ uint64_t buf[2][kRegSort];
     __shared__ int32_t segb[kRegSort + 1];  // segment starts
@@ -1309,7 +1310,73 @@ __global__ __launch_bounds__(1024) void k_small_insert(
     if (t == 0) {
         nscan[B2] = grow;
         *nmoved = moved;
+        if (ctl) {  // k_pool_reserve
+            ctl[kCtlGrow] = (unsigned long long)grow;
+            const unsigned long long used = ctl[kCtlPoolUsed];
+            if (used + (unsigned long long)grow <= ctl[kCtlPoolCap]) {
+                ctl[kCtlFits] = 1;
+                ctl[kCtlBase] = used;
+                ctl[kCtlPoolUsed] = used + (unsigned long long)grow;
+            } else {
+                ctl[kCtlFits] = 0;
+            }
+        }
     }
+}
+
+// k_set_flags for the deletions of a small batch (n <= 1024), and their
+// count (the present pairs among the first *nd) into *ndeleted: one block.
+__global__ __launch_bounds__(1024) void k_small_set_flags(
+    const int64_t *__restrict__ pos_ab, const int64_t *__restrict__ pos_ba,
+    const int32_t *__restrict__ sel, int64_t n, const int64_t *__restrict__ nd, uint8_t val,
+    uint8_t *__restrict__ flag, int64_t *__restrict__ ndeleted)
+{
+    __shared__ int32_t ws[1024 / kWave];
+    const int t = threadIdx.x;
+    const int32_t on = t < bound(n, nd) && sel[t];
+    if (on) {
+        if (pos_ab[t] >= 0) flag[pos_ab[t]] = val;
+        if (pos_ba[t] >= 0) flag[pos_ba[t]] = val;
+    }
+    int32_t c = on;
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+    if ((t & (kWave - 1)) == 0) ws[t / kWave] = c;
+    __syncthreads();
+    if (t == 0) {
+        int64_t tot = 0;
+        for (int q = 0; q < 1024 / kWave; ++q) tot += ws[q];
+        *ndeleted = tot;
+    }
+}
+
+// k_delta_items + its exclusive scan into pre[0..n] + k_zero_u32 for
+// n <= 1024 edges: one block.
+__global__ __launch_bounds__(1024) void k_small_delta_prep(
+    const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, const int32_t *__restrict__ sel,
+    int64_t n, const int64_t *__restrict__ nd, const unsigned long long *__restrict__ ctl,
+    const int32_t *__restrict__ len, int64_t *__restrict__ pre, uint32_t *__restrict__ ecnt)
+{
+    __shared__ int64_t ws[1024 / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    const int64_t lim = (ctl && !ctl[kCtlFits]) ? 0 : bound(n, nd);
+    const int64_t it = (t < lim && (sel == nullptr || sel[t])) ? min(len[pa[t]], len[pb[t]]) : 0;
+    if (t < n) ecnt[t] = 0u;
+    int64_t c = it;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int64_t y = __shfl_up(c, d, kWave);
+        if (lane >= d) c += y;
+    }
+    if (lane == kWave - 1) ws[wid] = c;
+    __syncthreads();
+    int64_t before = 0, total = 0;
+    for (int q = 0; q < 1024 / kWave; ++q) {
+        before += q < wid ? ws[q] : 0;
+        total += ws[q];
+    }
+    if (t < n) pre[t] = before + c - it;
+    if (t == 0) pre[n] = total;
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n)
@@ -1337,10 +1404,15 @@ int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel
     int32_t *items = g.lbuf[0].as<int32_t>();
     int64_t *pre = g.lbuf[1].as<int64_t>();
     uint32_t *ecnt = g.lbuf[2].as<uint32_t>();
-    hipLaunchKernelGGL(k_delta_items, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, sel, n,
-                       nd, ctl, g.st.len.as<int32_t>(), items);
-    TC_TRY(scan_counts_i32(items, pre, n, g.tmp, s));
-    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, ecnt, n);
+    if (n <= kSmallPairs) {
+        hipLaunchKernelGGL(k_small_delta_prep, dim3(1), dim3(1024), 0, s, pa, pb, sel, n, nd, ctl,
+                           g.st.len.as<int32_t>(), pre, ecnt);
+    } else {
+        hipLaunchKernelGGL(k_delta_items, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, sel,
+                           n, nd, ctl, g.st.len.as<int32_t>(), items);
+        TC_TRY(scan_counts_i32(items, pre, n, g.tmp, s));
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, ecnt, n);
+    }
     // a grid by the batch size (the item total is known on the device only;
     // the waves loop over the chunks)
     const int dblocks = (int)std::min<int64_t>(kDeltaBlocks, std::max<int64_t>(16, n / 4));
@@ -1508,7 +1580,8 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     };
     // the merges and the insertion's delta: run again after the pool grew
     auto merge_tail = [&](bool first, hipStream_t st) -> int {
-        hipLaunchKernelGGL(k_pool_reserve, dim3(1), dim3(64), 0, st, nscan, B2, ctl);
+        if (!(small && first))  // (k_small_insert reserves on the first run)
+            hipLaunchKernelGGL(k_pool_reserve, dim3(1), dim3(64), 0, st, nscan, B2, ctl);
         // the long rows first: a row merged by the wave kernel may grow past
         // kBigRow, and the block kernel, which picks its rows by their length,
         // would then merge it again
@@ -1556,8 +1629,12 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         hipLaunchKernelGGL(k_find_pairs, dim3(gB), dim3(256), 0, st, da, db, Bc, nD,
                            g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                            pab, pba, pres, directed ? 1 : 0);
-        hipLaunchKernelGGL(k_set_flags, dim3(gB), dim3(256), 0, st, pab, pba, pres, Bc, nD, (uint8_t)1,
-                           g.flagpool.as<uint8_t>());
+        if (small)  // (with the count of deleted pairs)
+            hipLaunchKernelGGL(k_small_set_flags, dim3(1), dim3(1024), 0, st, pab, pba, pres, Bc, nD,
+                               (uint8_t)1, g.flagpool.as<uint8_t>(), ndeleted);
+        else
+            hipLaunchKernelGGL(k_set_flags, dim3(gB), dim3(256), 0, st, pab, pba, pres, Bc, nD,
+                               (uint8_t)1, g.flagpool.as<uint8_t>());
         if (count) {
             TC_HIP(rec(9, st));
             TC_TRY(run_delta(g, da, db, pres, Bc, nD, nullptr, -1, out, acc + 2));
@@ -1579,8 +1656,10 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
                            g.vmark.as<int32_t>());
         TC_HIP(hipGetLastError());
         // deleted = present pairs (pres is 0 past the deletions)
-        TC_TRY(scan_counts_i32(pres, cnt, Bc, g.tmp, st));
-        hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, st, cnt + Bc, ndeleted, (int64_t)1);
+        if (!small) {
+            TC_TRY(scan_counts_i32(pres, cnt, Bc, g.tmp, st));
+            hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, st, cnt + Bc, ndeleted, (int64_t)1);
+        }
         if (track_sym)  // every deleted (a,b) needs (b,a) gone as well
             hipLaunchKernelGGL(k_reverse_check, dim3(grid_for(Bc, 256, 4096)), dim3(256), 0, st, da, db,
                                pres, Bc, nD, nullptr, g.st.pool.as<int32_t>(),
@@ -1593,7 +1672,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
             hipLaunchKernelGGL(k_small_insert, dim3(1), dim3(1024), 0, st, ia, ib, pres, Bc, nI, bv,
                                directed ? 1 : 0, pad, g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
                                g.grow_mul_x16, g.grow_add, absent, ndir, dk, sv, sbeg, hpos + B2, need,
-                               nscan, nmoved);
+                               nscan, nmoved, ctl);
         } else {
             hipLaunchKernelGGL(k_not, dim3(gB), dim3(256), 0, st, pres, Bc, nI, absent);
             TC_TRY(scan_counts_i32(absent, sidx, Bc, g.tmp, st));
