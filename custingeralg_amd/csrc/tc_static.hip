@@ -248,13 +248,29 @@ __device__ inline int32_t wave_row_of(const int64_t *__restrict__ coff,
     return row_search(coff, e, rfirst, rlast);
 }
 
-// crow[c] = row holding compact entry 64c.
+// crow[c] = row holding compact entry 64c.  A row holds the starts of chunks
+// ceil(a/64) .. ceil(b/64)-1 ([a, b) its entries); a row with more than 8 of
+// them is written by its whole wave.
 __global__ void k_chunk_rows(const int64_t *__restrict__ coff, int32_t nv, int32_t *__restrict__ crow)
 {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nv;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t a = coff[r], b = coff[r + 1];
-        for (int64_t c = (a + kWave - 1) / kWave; c * kWave < b; ++c) crow[c] = (int32_t)r;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); r0 < nv;
+         r0 += stride) {
+        const int64_t r = r0 + lane;
+        int64_t c0 = 0, c1 = 0;
+        if (r < nv) {
+            c0 = ceil_div(coff[r], kWave);
+            c1 = ceil_div(coff[r + 1], kWave);
+        }
+        const bool wide = c1 - c0 > 8;
+        if (!wide)
+            for (int64_t c = c0; c < c1; ++c) crow[c] = (int32_t)r;
+        for (uint64_t wm = __ballot(wide); wm; wm &= wm - 1) {
+            const int j = __ffsll((unsigned long long)wm) - 1;
+            const int64_t d0 = readlane_i64(c0, j), d1 = readlane_i64(c1, j);
+            for (int64_t c = d0 + lane; c < d1; c += kWave) crow[c] = (int32_t)(r0 + j);
+        }
     }
 }
 
@@ -510,10 +526,13 @@ __global__ void k_orient_write_plus_rank(const int64_t *__restrict__ coff,
         const int64_t e = c * kWave + lane;
         const uint64_t m = mask[c];
         if (!m) continue;  // (the whole wave)
-        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
-        if (e >= nnz || !((m >> lane) & 1ull)) continue;
+        const bool mine = e < nnz && ((m >> lane) & 1ull);
         const int32_t k = __popcll(m & lt);
-        padj[pdst[a] + cscan[c] + k] = tr[c * kWave + k];
+        // the entry's rank and the chunk's offset are in flight during the row search
+        const int32_t rb = mine ? tr[c * kWave + k] : 0;
+        const int64_t cs = cscan[c];
+        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
+        if (mine) padj[pdst[a] + cs + k] = rb;
     }
 }
 
@@ -1310,8 +1329,7 @@ struct PlanParams {
 __device__ inline int32_t est_items(int32_t L, int32_t pl)
 {
     // cheapest of streaming the task list (pl) and searching the owner list (L*log pl)
-    int32_t lg = 1;
-    while ((1 << lg) < pl + 1) ++lg;
+    const int32_t lg = max(1, 32 - __clz(pl));  // smallest lg >= 1 with 2^lg > pl
     int64_t bs = (int64_t)L * lg;
     return (int32_t)min<int64_t>(pl, bs) + 4;
 }
@@ -1340,6 +1358,8 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         int64_t e = c * kWave + lane;
+        // the task record does not depend on the row: in flight during the search
+        const int32_t tl = (ORIENTED && e < nnz) ? tasks[e].len() : 0;
         int32_t b = wave_row_of(coff, crow, nv, c, nnz);
         uint32_t w = 0;
         if (e < nnz) {
@@ -1350,7 +1370,7 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
                 bool ok;
                 if (ORIENTED) {
                     // only the suffix of N+(a) ranked above the owner can hit
-                    pl = tasks[e].len();
+                    pl = tl;
                     ok = pl > 0;
                 } else {
                     const int32_t a = adj[vb.rb + (e - coff[b])];
@@ -1479,8 +1499,8 @@ __global__ void k_plus_row_sum(const int64_t *__restrict__ pcoff, const int32_t 
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
          c += wstride) {
         const int64_t e = c * kWave + lane;
+        uint32_t x = e < nplus ? pcnt[e] : 0u;  // in flight during the row search
         const int32_t r = wave_row_of(pcoff, pcrow, nv, c, nplus);
-        uint32_t x = e < nplus ? pcnt[e] : 0u;
 #pragma unroll
         for (int d = 1; d < kWave; d <<= 1) {
             const uint32_t y = __shfl_up(x, d, kWave);
