@@ -46,6 +46,28 @@ __device__ inline int64_t find_pos(const int32_t *__restrict__ pool, int64_t bas
     return (lo < n && pool[base + lo] == key) ? base + lo : -1;
 }
 
+// find_pos for two keys in two rows, the searches in lockstep so that their
+// loads are in flight together.
+__device__ inline void find_pos2(const int32_t *__restrict__ pool, int64_t b1, int32_t n1, int32_t k1,
+                                 int64_t b2, int32_t n2, int32_t k2, int64_t &x1, int64_t &x2)
+{
+    int32_t lo1 = 0, c1 = n1, lo2 = 0, c2 = n2;  // lower bounds in [lo, lo + c]
+    while (c1 > 0 || c2 > 0) {
+        const int32_t h1 = c1 >> 1, h2 = c2 >> 1;
+        const int32_t v1 = c1 > 0 ? pool[b1 + lo1 + h1] : 0;
+        const int32_t v2 = c2 > 0 ? pool[b2 + lo2 + h2] : 0;
+        if (c1 > 0) {
+            if (v1 < k1) { lo1 += h1 + 1; c1 -= h1 + 1; } else { c1 = h1; }
+        }
+        if (c2 > 0) {
+            if (v2 < k2) { lo2 += h2 + 1; c2 -= h2 + 1; } else { c2 = h2; }
+        }
+    }
+    const int32_t e1 = lo1 < n1 ? pool[b1 + lo1] : 0, e2 = lo2 < n2 ? pool[b2 + lo2] : 0;
+    x1 = (lo1 < n1 && e1 == k1) ? b1 + lo1 : -1;
+    x2 = (lo2 < n2 && e2 == k2) ? b2 + lo2 : -1;
+}
+
 // Bound of a grid-stride loop: n, or the count *nd the device computed when
 // one is given.  The batch sequence sizes its launches by the batch size and
 // reads the counts its own kernels produced, so it needs no host read-back.
@@ -171,8 +193,13 @@ __global__ void k_find_pairs(const int32_t *__restrict__ pa, const int32_t *__re
             continue;
         }
         int32_t a = pa[i], b = pb[i];
-        int64_t x = find_pos(pool, boff[a], len[a], b);
-        int64_t y = directed ? -1 : find_pos(pool, boff[b], len[b], a);
+        int64_t x, y;
+        if (directed) {
+            x = find_pos(pool, boff[a], len[a], b);
+            y = -1;
+        } else {
+            find_pos2(pool, boff[a], len[a], b, boff[b], len[b], a, x, y);
+        }
         pos_ab[i] = x;
         pos_ba[i] = y;
         present[i] = (x >= 0 && (directed || y >= 0)) ? 1 : 0;
