@@ -51,6 +51,9 @@ void DevBuf::release()
 int Graph::ensure_events()
 {
     if (events_ready) return TC_OK;
+    if (!hcounts)
+        TC_HIP(hipHostMalloc(reinterpret_cast<void **>(&hcounts), 32 * sizeof(unsigned long long),
+                             hipHostMallocDefault));
     for (auto &e : ev) TC_HIP(hipEventCreate(&e));
     events_ready = true;
     return TC_OK;
@@ -531,6 +534,7 @@ void graph_destroy(Graph *g)
     for (DevBuf &b : g->lbuf) b.release();
     if (g->events_ready)
         for (auto &e : g->ev) (void)hipEventDestroy(e);
+    if (g->hcounts) (void)hipHostFree(g->hcounts);
     g->drop_capture();
     if (g->bexec) (void)hipGraphExecDestroy(g->bexec);
     if (g->bgraph) (void)hipGraphDestroy(g->bgraph);

@@ -162,6 +162,8 @@ struct Graph {
     hipEvent_t ev[16] = {};
     bool events_ready = false;
     int ensure_events();
+    // pinned host words the batch sequence's counts are copied to (32)
+    unsigned long long *hcounts = nullptr;
     // store version: bumped by every update, so that plans and recorded
     // counts of an older store are never reused
     uint64_t version = 0;

@@ -19,6 +19,7 @@
 #include "tc_internal.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace tc {
 
@@ -95,20 +96,46 @@ __global__ void k_batch_check(const int32_t *__restrict__ u, const int32_t *__re
 
 // The caller's batch copied into the handle's own arrays, padded with zeros
 // from B to Bc, and B kept on the device: a recorded batch sequence reads
-// only these.
+// only these.  The ids are checked on the way (k_batch_check's counts into
+// bad[0], bad[1], zeroed before): the front kernels see an empty batch when
+// the check fails, so an invalid batch changes nothing and the host learns
+// of it from the counts it reads at the end.
 __global__ void k_stage_batch(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
-                              const uint8_t *__restrict__ del, int64_t B, int64_t Bc,
+                              const uint8_t *__restrict__ del, int64_t B, int64_t Bc, int32_t nv,
                               int32_t *__restrict__ su, int32_t *__restrict__ sv,
-                              uint8_t *__restrict__ sd, int64_t *__restrict__ nreal)
+                              uint8_t *__restrict__ sd, int64_t *__restrict__ nreal,
+                              unsigned long long *__restrict__ bad)
 {
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 == 0) *nreal = B;
+    unsigned long long oor = 0, loops = 0;
     for (int64_t i = i0; i < Bc; i += (int64_t)gridDim.x * blockDim.x) {
         const bool in = i < B;
-        su[i] = in ? u[i] : 0;
-        sv[i] = in ? v[i] : 0;
+        const int32_t a = in ? u[i] : 0, b = in ? v[i] : 0;
+        if (in) {
+            if (a < 0 || a >= nv || b < 0 || b >= nv) ++oor;
+            else if (a == b) ++loops;
+        }
+        su[i] = a;
+        sv[i] = b;
         sd[i] = (in && del) ? del[i] : 0;
     }
+    for (int d = 32; d > 0; d >>= 1) {
+        oor += __shfl_xor(oor, d);
+        loops += __shfl_xor(loops, d);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (oor) atomicAdd(&bad[0], oor);
+        if (loops) atomicAdd(&bad[1], loops);
+    }
+}
+
+// The staged batch's real length, or 0 when k_stage_batch found ids out of
+// range (or self-loops, in an undirected batch).
+__device__ inline int64_t staged_len(int64_t Bc, const int64_t *nreal,
+                                     const unsigned long long *bad, int directed)
+{
+    return (bad[0] || (!directed && bad[1])) ? 0 : bound(Bc, nreal);
 }
 
 // key = (min << (bv+1)) | (max << 1) | is_delete; a directed batch keeps
@@ -119,11 +146,13 @@ __global__ void k_stage_batch(const int32_t *__restrict__ u, const int32_t *__re
 // skip, above every real key, and they take no part in what follows.
 __global__ void k_batch_keys(const int32_t *__restrict__ u, const int32_t *__restrict__ v,
                              const uint8_t *__restrict__ del, int64_t B,
-                             const int64_t *__restrict__ nreal, uint64_t skip, int bv, int directed,
-                             uint64_t *__restrict__ keys, unsigned long long *__restrict__ ndel)
+                             const int64_t *__restrict__ nreal,
+                             const unsigned long long *__restrict__ bad, uint64_t skip, int bv,
+                             int directed, uint64_t *__restrict__ keys,
+                             unsigned long long *__restrict__ ndel)
 {
     unsigned long long nd = 0;
-    const int64_t lim = bound(B, nreal);
+    const int64_t lim = staged_len(B, nreal, bad, directed);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < B;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (i >= lim) {
@@ -1157,16 +1186,16 @@ __device__ inline void scan_rows_2048(int64_t x0, int64_t x1, int64_t &e0, int64
 // deletion requests added to ndel[0].
 __global__ __launch_bounds__(1024) void k_small_front(
     const int32_t *__restrict__ u, const int32_t *__restrict__ v, const uint8_t *__restrict__ del,
-    int64_t Bc, const int64_t *__restrict__ nreal, uint64_t skip, int bv, int directed,
-    int32_t *__restrict__ ia, int32_t *__restrict__ ib, int32_t *__restrict__ da,
-    int32_t *__restrict__ db, int64_t *__restrict__ nI, int64_t *__restrict__ nD,
-    unsigned long long *__restrict__ ndel)
+    int64_t Bc, const int64_t *__restrict__ nreal, const unsigned long long *__restrict__ bad,
+    uint64_t skip, int bv, int directed, int32_t *__restrict__ ia, int32_t *__restrict__ ib,
+    int32_t *__restrict__ da, int32_t *__restrict__ db, int64_t *__restrict__ nI,
+    int64_t *__restrict__ nD, unsigned long long *__restrict__ ndel)
 {
     __shared__ uint64_t buf[2][kRegSort];
     __shared__ int32_t ws[16];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int i0 = 128 * wid + lane, i1 = i0 + 64;
-    const int64_t lim = bound(Bc, nreal);
+    const int64_t lim = staged_len(Bc, nreal, bad, directed);
     unsigned long long nd = 0;
     auto key = [&](int64_t i) -> uint64_t {
         if (i >= Bc) return ~0ull;  // past the batch: sorts last
@@ -1510,22 +1539,12 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     TC_TRY(g.ensure_events());
     TC_TRY(g.accum.ensure(32 * sizeof(unsigned long long)));
     unsigned long long *acc = g.accum.as<unsigned long long>();
+    // (the batch's ids are checked by k_stage_batch, in the sequence)
     hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, s, acc, 32);
-    hipLaunchKernelGGL(k_batch_check, dim3(grid_for(B, 256, 4096)), dim3(256), 0, s, u, v, B, nv, acc);
     TC_HIP(hipGetLastError());
-    unsigned long long hb[2];
-    TC_HIP(hipMemcpyAsync(hb, acc, sizeof(hb), hipMemcpyDeviceToHost, s));
-    TC_HIP(hipStreamSynchronize(s));
-    if (hb[0] || (!directed && hb[1])) {
-        set_last_error(TC_ERR_INVALID, "batch has " + std::to_string(hb[0]) +
-                                           " out-of-range ids and " + std::to_string(hb[1]) +
-                                           " self-loops; nothing applied");
-        return TC_ERR_INVALID;
-    }
     // a directed batch keeps a symmetric store's flag only if every touched
     // pair's reverse ends in the same state (acc[6] counts those that do not)
     const bool track_sym = directed && g.symmetric == 1;
-    if (directed && hb[1]) g.self_loops = -1;
     TC_TRY(ensure_flags(g));
 
     int bv = 1;
@@ -1594,8 +1613,8 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         TC_TRY(g.tmp4.ensure((size_t)std::max(nv, 1) * sizeof(uint32_t)));
         out = g.tmp4.as<uint32_t>();
     }
-    hipLaunchKernelGGL(k_stage_batch, dim3(gB), dim3(256), 0, s, u, v, is_del, B, Bc, su, svv, sdel,
-                       nreal);
+    hipLaunchKernelGGL(k_stage_batch, dim3(gB), dim3(256), 0, s, u, v, is_del, B, Bc, nv, su, svv,
+                       sdel, nreal, acc);
     hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, s, ctl, (unsigned long long)g.st.pool_used,
                        (unsigned long long)(g.st.pool.bytes / sizeof(int32_t)));
     TC_HIP(hipGetLastError());
@@ -1639,11 +1658,11 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     auto body = [&](hipStream_t st) -> int {
         // acc[5]: deletion requests of the raw batch
         if (small) {
-            hipLaunchKernelGGL(k_small_front, dim3(1), dim3(1024), 0, st, su, svv, sdel, Bc, nreal, skip,
-                               bv, directed ? 1 : 0, ia, ib, da, db, pi + Bc, pd + Bc, acc + 5);
+            hipLaunchKernelGGL(k_small_front, dim3(1), dim3(1024), 0, st, su, svv, sdel, Bc, nreal, acc,
+                               skip, bv, directed ? 1 : 0, ia, ib, da, db, pi + Bc, pd + Bc, acc + 5);
         } else {
-            hipLaunchKernelGGL(k_batch_keys, dim3(gB), dim3(256), 0, st, su, svv, sdel, Bc, nreal, skip,
-                               bv, directed ? 1 : 0, keys, acc + 5);
+            hipLaunchKernelGGL(k_batch_keys, dim3(gB), dim3(256), 0, st, su, svv, sdel, Bc, nreal, acc,
+                               skip, bv, directed ? 1 : 0, keys, acc + 5);
             TC_TRY(radix_sort_u64(keys, nullptr, Bc, bits, g.sb[1], g.sb[2], g.tmp, st));
             hipLaunchKernelGGL(k_unique_split, dim3(gB), dim3(256), 0, st, keys, Bc, skip, fi, fd);
             TC_TRY(scan_counts_i32(fi, pi, Bc, g.tmp, st));
@@ -1782,12 +1801,20 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         }
     }
     if (!replayed) TC_TRY(body(s));
+    // all counts at once: the id check acc[0..1], acc[2..6], the pool
+    // reservation, the new entries, the deletions, the moved rows
+    unsigned long long h[27];
+    TC_HIP(hipMemcpyAsync(g.hcounts, acc, sizeof(h), hipMemcpyDeviceToHost, s));
     TC_HIP(hipEventRecord(g.ev[13], s));
     TC_HIP(hipEventSynchronize(g.ev[13]));
-    // all counts at once: acc[2..6], the pool reservation, the new entries,
-    // the deletions, the moved rows
-    unsigned long long h[27];
-    TC_HIP(hipMemcpy(h, acc, sizeof(h), hipMemcpyDeviceToHost));
+    std::memcpy(h, g.hcounts, sizeof(h));
+    if (h[0] || (!directed && h[1])) {
+        set_last_error(TC_ERR_INVALID, "batch has " + std::to_string(h[0]) +
+                                           " out-of-range ids and " + std::to_string(h[1]) +
+                                           " self-loops; nothing applied");
+        return TC_ERR_INVALID;
+    }
+    if (directed && h[1]) g.self_loops = -1;
     float ms = 0;
     double ms_delta = 0;
     if (count && !replayed) {

@@ -378,6 +378,20 @@ def test_recorded_batches(slack):
             # 900..1000 pairs: one size class (1024); hub-heavy R-MAT inserts
             bu, bv, dl = _batch(rng, n, cur_off, cur_adj, 800 + int(rng.integers(0, 50)),
                                 100 + int(rng.integers(0, 50)), rmat_scale=13 if bi % 2 else None)
+            if bi in (4, 7, 8):
+                # an invalid batch of the same class (one id out of range, or
+                # one self-loop) replays the recorded sequence and changes
+                # nothing: the checks below see only the valid batches
+                xu, xv = bu.copy(), bv.copy()
+                k = len(xu) // 2
+                if bi == 4:
+                    xv[k] = n + 5
+                elif bi == 7:
+                    xu[k] = -1
+                else:
+                    xv[k] = xu[k]
+                with pytest.raises(T.TCError):
+                    T.batch_apply(g, xu, xv, dl, delta, schedule=sched)
             dg, st = T.batch_apply(g, bu, bv, dl, delta, schedule=sched)
             dg2, st2 = T.batch_apply(twin, bu, bv, dl, dtwin)
             replayed.append(st["replayed"])
