@@ -350,7 +350,8 @@ def bench_stream(T, args) -> dict:
             "value": n_all / t_all, "unit": "updates/s", "ms_per_step": t_all / len(sizes) * 1e3,
             "higher_is_better": True, "dtype": "int32",
             "config": {"workload": f"streaming TC, R-MAT scale {scale} ef {ef} seed {seed} base, "
-                                   "20 batches 5x10^3..5x10^6, 80% inserts / 20% deletes",
+                                   "20 batches (5 each of 10^3, 10^4, 10^5 and 10^6 pairs), "
+                                   "80% inserts / 20% deletes",
                        "per_batch_size": rates, "final_global_count": total,
                        "recorded_batches": not args.no_capture,
                        "parity": ("running total equals a GPU recount and the oracle's T "
