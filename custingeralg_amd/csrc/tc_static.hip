@@ -248,6 +248,32 @@ __device__ inline int32_t wave_row_of(const int64_t *__restrict__ coff,
     return row_search(coff, e, rfirst, rlast);
 }
 
+// wave_row_of for two chunks c1 and c2, their binary searches in lockstep so
+// that the loads of both are in flight together.
+__device__ inline void wave_rows_of2(const int64_t *__restrict__ coff,
+                                     const int32_t *__restrict__ crow, int32_t nv, int64_t c1,
+                                     int64_t c2, int64_t nnz, int32_t &r1, int32_t &r2)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nchunks = ceil_div(nnz, kWave);
+    const int64_t e1 = min(c1 * kWave + lane, min(c1 * kWave + kWave, nnz) - 1);
+    const int64_t e2 = min(c2 * kWave + lane, min(c2 * kWave + kWave, nnz) - 1);
+    int32_t lo1 = crow[c1], hi1 = (c1 + 1 < nchunks) ? crow[c1 + 1] : nv - 1;
+    int32_t lo2 = crow[c2], hi2 = (c2 + 1 < nchunks) ? crow[c2 + 1] : nv - 1;
+    while (lo1 < hi1 || lo2 < hi2) {
+        const int32_t m1 = lo1 + ((hi1 - lo1 + 1) >> 1), m2 = lo2 + ((hi2 - lo2 + 1) >> 1);
+        const int64_t v1 = lo1 < hi1 ? coff[m1] : 0, v2 = lo2 < hi2 ? coff[m2] : 0;
+        if (lo1 < hi1) {
+            if (v1 <= e1) lo1 = m1; else hi1 = m1 - 1;
+        }
+        if (lo2 < hi2) {
+            if (v2 <= e2) lo2 = m2; else hi2 = m2 - 1;
+        }
+    }
+    r1 = lo1;
+    r2 = lo2;
+}
+
 // crow[c] = row holding compact entry 64c.  A row holds the starts of chunks
 // ceil(a/64) .. ceil(b/64)-1 ([a, b) its entries); a row with more than 8 of
 // them is written by its whole wave.
@@ -1355,12 +1381,9 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     // per-lane bin statistics, reduced once at the end
     unsigned long long st_t[3] = {0, 0, 0}, st_b[3] = {0, 0, 0}, st_i[3] = {0, 0, 0};
-    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
-         c += wstride) {
-        int64_t e = c * kWave + lane;
-        // the task record does not depend on the row: in flight during the search
-        const int32_t tl = (ORIENTED && e < nnz) ? tasks[e].len() : 0;
-        int32_t b = wave_row_of(coff, crow, nv, c, nnz);
+    // one chunk: c, its row b per lane, the task suffix length tl (oriented)
+    auto chunk = [&](int64_t c, int32_t b, int32_t tl) {
+        const int64_t e = c * kWave + lane;
         uint32_t w = 0;
         if (e < nnz) {
             VInfo vb = vi[b];
@@ -1403,6 +1426,20 @@ __global__ void k_owner_work(const int64_t *__restrict__ coff, const int32_t *__
         int32_t bnext = __shfl_down(b, 1, kWave);
         bool tail = (lane == kWave - 1) || (bnext != b) || (e + 1 >= nnz);
         if (e < nnz && tail && x) atomicAdd(&work[b], (unsigned long long)x);
+    };
+    // two chunks per pass (c and c + wstride): the task records of both and
+    // their row searches in flight together
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += 2 * wstride) {
+        const int64_t c2 = c + wstride;
+        const bool two = c2 < nchunks;  // (wave-uniform)
+        const int64_t e = c * kWave + lane, e2 = c2 * kWave + lane;
+        const int32_t tl = (ORIENTED && e < nnz) ? tasks[e].len() : 0;
+        const int32_t tl2 = (ORIENTED && two && e2 < nnz) ? tasks[e2].len() : 0;
+        int32_t b, b2;
+        wave_rows_of2(coff, crow, nv, c, two ? c2 : c, nnz, b, b2);
+        chunk(c, b, tl);
+        if (two) chunk(c2, b2, tl2);
     }
     if (bstats) {
         // bstats layout: tasks/bytes pairs per bin, then items per bin
@@ -1496,11 +1533,7 @@ __global__ void k_plus_row_sum(const int64_t *__restrict__ pcoff, const int32_t 
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nchunks = ceil_div(nplus, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
-         c += wstride) {
-        const int64_t e = c * kWave + lane;
-        uint32_t x = e < nplus ? pcnt[e] : 0u;  // in flight during the row search
-        const int32_t r = wave_row_of(pcoff, pcrow, nv, c, nplus);
+    auto chunk = [&](int64_t e, int32_t r, uint32_t x) {
 #pragma unroll
         for (int d = 1; d < kWave; d <<= 1) {
             const uint32_t y = __shfl_up(x, d, kWave);
@@ -1510,6 +1543,20 @@ __global__ void k_plus_row_sum(const int64_t *__restrict__ pcoff, const int32_t 
         const int32_t rn = __shfl_down(r, 1, kWave);
         const bool tail = lane == kWave - 1 || rn != r || e + 1 >= nplus;
         if (e < nplus && tail && x) atomicAdd(&outr[r], 2u * x);
+    };
+    // two chunks per pass (c and c + wstride), their counts and row searches
+    // in flight together
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
+         c += 2 * wstride) {
+        const int64_t c2 = c + wstride;
+        const bool two = c2 < nchunks;  // (wave-uniform)
+        const int64_t e = c * kWave + lane, e2 = c2 * kWave + lane;
+        const uint32_t x = e < nplus ? pcnt[e] : 0u;
+        const uint32_t x2 = (two && e2 < nplus) ? pcnt[e2] : 0u;
+        int32_t r, r2;
+        wave_rows_of2(pcoff, pcrow, nv, c, two ? c2 : c, nplus, r, r2);
+        chunk(e, r, x);
+        if (two) chunk(e2, r2, x2);
     }
 }
 
