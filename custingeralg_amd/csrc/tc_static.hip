@@ -60,10 +60,11 @@ __device__ inline int64_t readlane_i64(int64_t x, int lane)
 
 // Bucketed two-choice hash table in LDS.  Bucket b holds up to four ids in
 // slots 4b..4b+3, filled as a prefix (inserts claim the first empty slot by
-// CAS), -1 when empty.  An id goes to the less full of its two buckets, and
-// only when both are full to the next non-full bucket after the first.  A
-// lookup reads both buckets (two 16-byte LDS reads) and loops over further
-// buckets only when both are full, which the load factor <= 1/2 makes rare.
+// CAS), -1 when empty.  An id goes to its first bucket, to its second when
+// the first is full, and when both are full to the next non-full bucket after
+// the first.  A lookup reads the first bucket (one 16-byte LDS read), the
+// second only when the first is full, and loops over further buckets only
+// when both are full; the load factor <= 1/2 makes each step rarer.
 struct BucketTable {
     int4 *HB;
     int nbb;  // log2(number of buckets)
@@ -76,10 +77,8 @@ struct BucketTable {
     {
         return nbb ? ((((uint32_t)w ^ 0x2C1B3C6Du) * 0x297A2D39u) >> (32 - nbb)) : 0u;
     }
-    __device__ inline int fill(const int4 x) const
-    {
-        return (x.x >= 0) + (x.y >= 0) + (x.z >= 0) + (x.w >= 0);
-    }
+    // A bucket's ids fill its slots in order (claim takes the lowest free
+    // one), so a bucket is full exactly when its last slot is taken.
     __device__ inline bool claim(uint32_t b, int32_t w) const
     {
         int *s = reinterpret_cast<int *>(&HB[b]);
@@ -88,13 +87,17 @@ struct BucketTable {
             if (atomicCAS(&s[k], -1, w) == -1) return true;
         return false;
     }
+    // First fit: the first bucket, then the second, then the buckets after
+    // the first in turn.  An id is therefore in its first bucket unless that
+    // bucket is full, and in its second unless both are (no id leaves the
+    // table while it is read).
     __device__ inline void insert(int32_t w) const
     {
         const uint32_t m = (1u << nbb) - 1;
-        const uint32_t x = b1(w), y = b2(w);
-        const bool first = fill(HB[x]) <= fill(HB[y]);
-        if (claim(first ? x : y, w)) return;
-        if (claim(first ? y : x, w)) return;
+        const uint32_t x = b1(w);
+        if (claim(x, w)) return;
+        const uint32_t y = b2(w);
+        if (claim(y, w)) return;
         for (uint32_t b = (x + 1) & m;; b = (b + 1) & m)
             if (claim(b, w)) return;
     }
@@ -105,14 +108,14 @@ struct BucketTable {
         const int32_t idx = m1 + 2 * m2 + 3 * m3;
         return (m0 | m1 | m2 | m3) ? (int32_t)(b << 2) + idx : -1;
     }
-    // Slot of w, or -1 (w < 0 is never present).
-    __device__ inline int32_t find(int32_t w) const
+    // The rest of a lookup whose first bucket vx (index x) is full and does
+    // not hold w: the second bucket, then the probe sequence if both are full.
+    __device__ inline int32_t find_rest(uint32_t x, int32_t w) const
     {
-        const uint32_t x = b1(w), y = b2(w);
-        const int4 vx = HB[x], vy = HB[y];
-        int32_t s = match(vx, x, w);
-        if (s < 0) s = match(vy, y, w);
-        if (s < 0 && w >= 0 && vx.w >= 0 && vy.w >= 0) {
+        const uint32_t y = b2(w);
+        const int4 vy = HB[y];
+        int32_t s = match(vy, y, w);
+        if (s < 0 && vy.w >= 0) {
             const uint32_t m = (1u << nbb) - 1;
             for (uint32_t b = (x + 1) & m;; b = (b + 1) & m) {
                 const int4 v = HB[b];
@@ -120,34 +123,58 @@ struct BucketTable {
                 if (s >= 0 || v.w < 0) break;
             }
         }
+        return s;
+    }
+    // Slot of w, or -1 (w < 0 is never present).
+    __device__ inline int32_t find(int32_t w) const
+    {
+        const uint32_t x = b1(w);
+        const int4 vx = HB[x];
+        int32_t s = match(vx, x, w);
+        if (s < 0 && w >= 0 && vx.w >= 0) s = find_rest(x, w);
         return w >= 0 ? s : -1;  // empty slots hold -1
     }
-    // Batched find: both buckets of every value are read before any is matched.
+    // Batched find: the first buckets of every value are read before any is
+    // matched; a second bucket is read only where the first is full.
     template <int U>
     __device__ inline void find_n(const int32_t *w, int32_t *slot) const
     {
-        int4 vx[U], vy[U];
-        uint32_t x[U], y[U];
+        int4 vx[U];
+        uint32_t x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             x[u] = b1(w[u]);
-            y[u] = b2(w[u]);
             vx[u] = HB[x[u]];
-            vy[u] = HB[y[u]];
+        }
+        int4 vy[U];
+        uint32_t y[U];
+        int32_t s[U];
+        bool need[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            s[u] = match(vx[u], x[u], w[u]);
+            need[u] = s[u] < 0 && w[u] >= 0 && vx[u].w >= 0;
+            y[u] = 0;
+            vy[u] = make_int4(-1, -1, -1, -1);
+            if (need[u]) {  // (these reads are in flight together)
+                y[u] = b2(w[u]);
+                vy[u] = HB[y[u]];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            int32_t s = match(vx[u], x[u], w[u]);
-            if (s < 0) s = match(vy[u], y[u], w[u]);
-            if (s < 0 && w[u] >= 0 && vx[u].w >= 0 && vy[u].w >= 0) {
-                const uint32_t m = (1u << nbb) - 1;
-                for (uint32_t b = (x[u] + 1) & m;; b = (b + 1) & m) {
-                    const int4 v = HB[b];
-                    s = match(v, b, w[u]);
-                    if (s >= 0 || v.w < 0) break;
+            if (need[u]) {
+                s[u] = match(vy[u], y[u], w[u]);
+                if (s[u] < 0 && vy[u].w >= 0) {
+                    const uint32_t m = (1u << nbb) - 1;
+                    for (uint32_t b = (x[u] + 1) & m;; b = (b + 1) & m) {
+                        const int4 v = HB[b];
+                        s[u] = match(v, b, w[u]);
+                        if (s[u] >= 0 || v.w < 0) break;
+                    }
                 }
             }
-            slot[u] = w[u] >= 0 ? s : -1;
+            slot[u] = w[u] >= 0 ? s[u] : -1;
         }
     }
     __device__ inline int32_t id_at(int32_t slot) const
