@@ -128,6 +128,20 @@ def test_invalid_batch_leaves_store_untouched():
         T.batch_apply(g, np.array([1, 2], np.int32), np.array([5, 2], np.int32))  # self-loop
     with pytest.raises(T.TCError):
         T.batch_apply(g, np.array([1], np.int32), np.array([1 << 20], np.int32))  # out of range
+    # a batch above the one-block size (1,024 pairs) takes the multi-block
+    # front kernels: one bad id among 3,000 valid pairs, then one self-loop
+    rng = np.random.default_rng(4)
+    n = len(off) - 1
+    bu = rng.integers(0, n, 3000).astype(np.int32)
+    bv = ((bu + 1 + rng.integers(0, n - 1, 3000)) % n).astype(np.int32)
+    for k, bad in ((1500, n), (2999, None)):
+        xu, xv = bu.copy(), bv.copy()
+        if bad is None:
+            xv[k] = xu[k]
+        else:
+            xv[k] = bad
+        with pytest.raises(T.TCError):
+            T.batch_apply(g, xu, xv)
     got = g.to_csr()
     assert np.array_equal(got[0], off) and np.array_equal(got[1], adj)
     g.close()
