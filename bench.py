@@ -272,8 +272,19 @@ def bench_static(T, args) -> dict:
                  "merge_equiv_bytes": st["bytes_alg_bin"], "strategy_tasks": st["strategy_tasks"],
                  "long_ms": st["ms_long"], "long_tasks": st["tasks_long"],
                  "long_items": st["items_long"], "long_stream_bytes": st["stream_bytes_long"],
-                 "ms_prepare": st["ms_prepare"], "ms_count": st["ms_count"]},
+                 "ms_prepare": st["ms_prepare"], "ms_count": st["ms_count"],
+                 "build": build_roofline(nnz, st["ms_prepare"])},
     }
+
+
+def build_roofline(nnz: int, ms: float) -> dict:
+    """The orientation build against the bytes any build must move: per store
+    entry its id and its neighbour's rank (4 + 4 B), per plus entry (half of
+    them) its N+ rank and its task record (4 + 8 B), 14 B per store entry."""
+    alg = 14 * nnz
+    gbs = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"ms": ms, "alg_bytes_per_entry": 14, "alg_bytes": alg, "achieved": gbs,
+            "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
 
 
 def _device_pairs(T):
