@@ -529,22 +529,32 @@ __global__ void k_orient_mask_rank(const int64_t *__restrict__ coff, const int32
     const int64_t nchunks = ceil_div(nnz, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // two chunks per pass (c and c + wstride): their row searches in
+    // lockstep, then both chunks' loads and rank lookups in flight together
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
-         c += wstride) {
-        const int64_t e = c * kWave + lane;
-        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
-        bool plus = false;
-        int32_t rb = 0;
-        if (e < nnz) {
-            rb = rnk[pool[boff[a] + (e - coff[a])]];
-            plus = rb > rnk[a];
-        }
-        const uint64_t m = __ballot(plus);
-        // the chunk's plus entries, compacted to the front of its 64 slots
+         c += 2 * wstride) {
+        const int64_t c2 = c + wstride;
+        const bool two = c2 < nchunks;  // (wave-uniform)
+        const int64_t e = c * kWave + lane, e2 = c2 * kWave + lane;
+        int32_t a, a2;
+        wave_rows_of2(coff, crow, nv, c, two ? c2 : c, nnz, a, a2);
+        const bool in = e < nnz, in2 = two && e2 < nnz;
+        const int32_t b = in ? pool[boff[a] + (e - coff[a])] : 0;
+        const int32_t b2 = in2 ? pool[boff[a2] + (e2 - coff[a2])] : 0;
+        const int32_t ra = in ? rnk[a] : 0, ra2 = in2 ? rnk[a2] : 0;
+        const int32_t rb = in ? rnk[b] : 0, rb2 = in2 ? rnk[b2] : 0;
+        const bool plus = in && rb > ra, plus2 = in2 && rb2 > ra2;
+        const uint64_t m = __ballot(plus), m2 = __ballot(plus2);
+        // each chunk's plus entries, compacted to the front of its 64 slots
         if (plus) tr[c * kWave + __popcll(m & lt)] = rb;
+        if (plus2) tr[c2 * kWave + __popcll(m2 & lt)] = rb2;
         if (lane == 0) {
             mask[c] = m;
             cnt[c] = __popcll(m);
+            if (two) {
+                mask[c2] = m2;
+                cnt[c2] = __popcll(m2);
+            }
         }
     }
 }
@@ -574,18 +584,26 @@ __global__ void k_orient_write_plus_rank(const int64_t *__restrict__ coff,
     const int64_t nchunks = ceil_div(nnz, kWave);
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / kWave);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    // two chunks per pass (c and c + wstride), their row searches in
+    // lockstep; the entries' ranks and the chunks' offsets are in flight
+    // during the searches
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < nchunks;
-         c += wstride) {
-        const int64_t e = c * kWave + lane;
-        const uint64_t m = mask[c];
-        if (!m) continue;  // (the whole wave)
+         c += 2 * wstride) {
+        const int64_t c2 = c + wstride;
+        const bool two = c2 < nchunks;  // (wave-uniform)
+        const int64_t e = c * kWave + lane, e2 = c2 * kWave + lane;
+        const uint64_t m = mask[c], m2 = two ? mask[c2] : 0ull;
+        if (!m && !m2) continue;  // (the whole wave)
         const bool mine = e < nnz && ((m >> lane) & 1ull);
-        const int32_t k = __popcll(m & lt);
-        // the entry's rank and the chunk's offset are in flight during the row search
+        const bool mine2 = e2 < nnz && ((m2 >> lane) & 1ull);
+        const int32_t k = __popcll(m & lt), k2 = __popcll(m2 & lt);
         const int32_t rb = mine ? tr[c * kWave + k] : 0;
-        const int64_t cs = cscan[c];
-        const int32_t a = wave_row_of(coff, crow, nv, c, nnz);
+        const int32_t rb2 = mine2 ? tr[c2 * kWave + k2] : 0;
+        const int64_t cs = m ? cscan[c] : 0, cs2 = m2 ? cscan[c2] : 0;
+        int32_t a, a2;
+        wave_rows_of2(coff, crow, nv, m ? c : c2, m2 ? c2 : c, nnz, a, a2);
         if (mine) padj[pdst[a] + cs + k] = rb;
+        if (mine2) padj[pdst[a2] + cs2 + k2] = rb2;
     }
 }
 
