@@ -11,7 +11,8 @@ global count.  Three legs run on the same graph, each K timed steps:
   fixed    built-in schedule, fixed degree-ratio strategy heuristic;
   tuned    autotuned bin thresholds / job grains, fixed heuristic;
   learned  autotuned schedule with the learned strategy table (MLP trained on
-           the device from intersection timings of R-MAT scale 18, 20, 22).
+           the device from in-situ count times of R-MAT graphs of other
+           seeds, custingeralg_amd/selector_model.json).
 
 value = nnz / step time of the learned leg; all three must give the same
 global count.  The roofline object describes the dominant count kernel by
@@ -187,7 +188,7 @@ def bench_static(T, args) -> dict:
     legs = {"fixed": leg("fixed", T.POLICY_FIXED)}
 
     # learned selector and tuned schedule: saved by custingeralg_amd.train_selector
-    # (trained on the GPU from R-MAT 18/20/22 timings), else trained here
+    # (trained on the GPU from in-situ count times), else trained here
     t0 = time.time()
     if os.path.exists(args.selector):
         sel = Lm.load_selector(args.selector)

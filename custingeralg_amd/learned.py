@@ -298,27 +298,28 @@ def insitu_sweep(g: cuStinger, reps: int = 3, strategies=(0, 1, 2)):
 def train_insitu(graphs, reps: int = 3, steps: int = 400, lr: float = 0.01, seed: int = 0,
                  min_gain: float = 0.01):
     """In-situ sweeps on each graph, then the model trained on the device on
-    the relative costs and loaded.  A group keeps the fixed heuristic's
-    strategy unless another one saved at least min_gain of the count on
-    average.  Returns (model, table, per-graph sweeps)."""
+    the relative costs and loaded.  Every cell's target is the fixed
+    heuristic's strategy (the others 0.5 above it), except in a group where
+    switching every cell to one strategy saved at least min_gain of the count
+    on average: there that strategy is the target, the others 10x their
+    measured extra cost above it (at least 0.5, wider than the network's fit
+    error).  Returns (model, table, losses, per-graph sweeps)."""
     sweeps = [insitu_sweep(g, reps) for g in graphs]
     base = fixed_table()
     x = cell_features()
-    # every cell: the fixed strategy cheapest by a margin unless measured otherwise
     tgt = np.full((TABLE_DIM * TABLE_DIM, 3), 0.5, np.float32)
     tgt[np.arange(TABLE_DIM * TABLE_DIM), base.ravel()] = 0.0
     for key in sweeps[0]:
         rows, cols = key
         rel = {st: float(np.mean([sw[key][st] for sw in sweeps])) for st in sweeps[0][key]}
-        b = int(base[rows[0], cols[0]])
         best = min(rel, key=rel.get)
-        if rel[best] > rel.get(b, 1.0) - min_gain:
-            best = b
+        if rel[best] > 1.0 - min_gain:
+            continue  # (the unmodified table is 1.0)
         for i in range(rows[0], rows[1] + 1):
             for j in range(cols[0], cols[1] + 1):
                 c = i * TABLE_DIM + j
                 for st, r in rel.items():
-                    tgt[c, st] = 0.0 if st == best else max(10.0 * (r - rel[best]), 0.1)
+                    tgt[c, st] = 0.0 if st == best else max(10.0 * (r - rel[best]), 0.5)
     model = CostModel(seed=seed)
     losses = model.fit(x, tgt, steps, lr)
     table = model.load()

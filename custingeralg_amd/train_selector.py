@@ -3,8 +3,10 @@
     python -m custingeralg_amd.train_selector [--out PATH] [--tune-scale 24] [--timings]
 
 By default the costs are measured in situ: whole counts of R-MAT graphs of
-scales 18, 20 and 22 (edge factor 16) with one group of strategy-table cells
-switched at a time (learned.train_insitu); with --timings, from one-wavefront
+scales 20, 22 and 24 (edge factor 16, seeds 100 + scale) with one group of
+strategy-table cells switched at a time (learned.train_insitu); a group leaves
+the fixed heuristic's strategy only for one that saved at least --min-gain of
+the count on average.  With --timings, the costs come from one-wavefront
 timings of sampled tasks instead.  The cost model is trained by the library's
 own FP32 kernels, and the schedule is autotuned on an R-MAT graph of the tune
 scale (edge factor 32, seed 5: a different graph from the benchmark's seed
@@ -27,9 +29,10 @@ from . import learned as Lm
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=Lm.default_model_path())
-    ap.add_argument("--scales", default="18,20,22")
+    ap.add_argument("--scales", default="20,22,24")
+    ap.add_argument("--min-gain", type=float, default=0.015)
     ap.add_argument("--samples", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--tune-scale", type=int, default=24)
     ap.add_argument("--tune-ef", type=int, default=32)
     ap.add_argument("--tune-seed", type=int, default=5)
@@ -45,7 +48,8 @@ def main(argv=None) -> int:
     else:
         graphs = [rmat_graph_device(s, 16, 100 + s) for s in scales]
         try:
-            model, table, losses, sweeps = Lm.train_insitu(graphs, steps=args.steps)
+            model, table, losses, sweeps = Lm.train_insitu(graphs, steps=args.steps,
+                                                            min_gain=args.min_gain)
         finally:
             for g in graphs:
                 g.close()
@@ -59,7 +63,8 @@ def main(argv=None) -> int:
         for k, v in rel.items():
             print(f"[train]   {k}: {v}", flush=True)
         source = {"costs": "in situ: whole counts with one cell group switched",
-                  "sweep_points": ncounts, "sweep_relative_time": rel}
+                  "sweep_points": ncounts, "min_gain": args.min_gain,
+                  "sweep_relative_time": rel}
         print(f"[train] in-situ sweeps of R-MAT {scales}: loss {losses[0]:.4f} -> "
               f"{losses[-1]:.4f} ({time.time() - t0:.1f}s)", flush=True)
     t0 = time.time()
