@@ -319,6 +319,58 @@ def test_hub_rows_merge_and_compact(slack, n, hub_deg):
     g.close()
 
 
+def test_large_batch_matches_two_halves():
+    """A batch of over 2^18 pairs, a third of them at six hub rows of 9,000+
+    entries, against the same batch applied to a twin store in two halves:
+    the same store and the same per-vertex delta, the running total equal to
+    a recount, and the store equal to the oracle's."""
+    off, adj = _hub_graph(65536, 6, 9000, 31)
+    n = len(off) - 1
+    assert np.diff(off).max() >= 4096
+    rng = np.random.default_rng(32)
+    deg = np.diff(off)
+    rows = np.repeat(np.arange(n, dtype=np.int32), deg)
+    # insertions: half uniform, half at the hubs; deletions: present edges
+    ni = 280000
+    iu = np.concatenate([rng.integers(0, n, ni // 2), rng.integers(0, 6, ni // 2)]).astype(np.int32)
+    iv = rng.integers(0, n, ni).astype(np.int32)
+    e = rng.choice(len(adj), 60000, replace=False)
+    keys_i = np.unique((np.minimum(iu, iv).astype(np.int64) << 32) | np.maximum(iu, iv))
+    keys_i = keys_i[(keys_i >> 32) != (keys_i & 0xFFFFFFFF)]
+    keys_d = np.unique((np.minimum(rows[e], adj[e]).astype(np.int64) << 32) |
+                       np.maximum(rows[e], adj[e]))
+    keys_i = keys_i[~np.isin(keys_i, keys_d)]
+    keys = np.concatenate([keys_i, keys_d])
+    u = (keys >> 32).astype(np.int32)
+    v = (keys & 0xFFFFFFFF).astype(np.int32)
+    dl = np.concatenate([np.zeros(len(keys_i), np.uint8), np.ones(len(keys_d), np.uint8)])
+    assert len(u) >= 1 << 18
+    g = T.cuStinger(off, adj)
+    twin = T.cuStinger(off, adj)
+    d1 = T.DeviceArray(n, np.uint32).zero()
+    d2 = T.DeviceArray(n, np.uint32).zero()
+    try:
+        _, total0, _ = T.static_count(g, per_vertex=False)
+        dg, _ = T.batch_apply(g, u, v, dl, d1)
+        perm = rng.permutation(len(u))
+        h = len(u) // 2
+        dga, _ = T.batch_apply(twin, u[perm[:h]], v[perm[:h]], dl[perm[:h]], d2)
+        dgb, _ = T.batch_apply(twin, u[perm[h:]], v[perm[h:]], dl[perm[h:]], d2)
+        assert dg == dga + dgb
+        assert np.array_equal(d1.to_host(), d2.to_host())
+        assert g.compare(twin) == {"size_mismatch": 0, "entry_mismatch": 0, "first_vertex": -1}
+        _, total1, _ = T.static_count(g, per_vertex=False)
+        assert total1 == total0 + dg
+        want_off, want_adj = _host_apply(n, off, adj, u, v, dl)
+        got_off, got_adj = g.to_csr()
+        assert np.array_equal(got_off, want_off) and np.array_equal(got_adj, want_adj)
+    finally:
+        d1.free()
+        d2.free()
+        g.close()
+        twin.close()
+
+
 @pytest.mark.parametrize("slack", [0, 100])
 def test_rows_grow_past_block_threshold(slack):
     """Rows just below the length from which a whole block merges them (1024)
