@@ -2072,7 +2072,9 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
             // Long hash tasks are streamed in 64-entry chunks across the whole
             // wave (stream_tasks): one batch of kBigUnroll loads per lane spans
             // several tasks, and hit counts come from ballots.
-            const bool is_big = strat == S_HASH && pl >= big_task_min<LMAX>();
+            // (a bin whose threshold no task reaches compiles no stream)
+            constexpr bool kStreams = big_task_min<LMAX>() < (1 << 30);
+            const bool is_big = kStreams && strat == S_HASH && pl >= big_task_min<LMAX>();
             if (!kChunkedLong) {
                 // largest bin: one long task at a time across the wave
                 uint64_t big = __ballot(is_big);
@@ -2104,7 +2106,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
                     h = wave_sum<uint32_t>(h);
                     if (lane == jb2) tcnt[wid][jb2] = h;
                 }
-            } else if (__ballot(is_big)) {
+            } else if (kStreams && __ballot(is_big)) {
                 const uint32_t h = stream_tasks<kBigUnroll, ORIENTED>(
                     A.adj, is_big ? pl : 0, pb, [&](const int32_t (&w)[kBigUnroll], uint32_t (&hc)[kBigUnroll]) {
                         int32_t slots[kBigUnroll];
@@ -2127,7 +2129,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
             for (int phase = 0; phase < 2; ++phase) {
                 const int want = phase == 0 ? S_HASH : S_BSEARCH;
                 const int32_t items =
-                    (strat == want && !(phase == 0 && pl >= big_task_min<LMAX>())) ? (phase == 0 ? pl : L) : 0;
+                    (strat == want && !(phase == 0 && kStreams && pl >= big_task_min<LMAX>())) ? (phase == 0 ? pl : L) : 0;
                 const int32_t incl = wave_incl_scan_i32(items);
                 wave_sync();
                 pre[wid][lane + 1] = incl;
