@@ -151,7 +151,7 @@ struct Graph {
     // grow_add (the update allocator's policy); 0: the next power of two
     int32_t grow_mul_x16 = 0;
     int32_t grow_add = 0;
-    DevBuf lbuf[3];  // load-balanced delta: items, their prefix, per-edge counts
+    DevBuf lbuf[5];  // delta: items, their prefix, per-edge counts, merge segments, their prefix
     bool degree_stale = false;
     // batched counting: graph boundaries and per-graph output (device)
     const int64_t *batch_voff = nullptr;

@@ -29,6 +29,12 @@ constexpr int kThreads = 256;
 constexpr int kBigRow = 1024;   // rows this long are compacted / merged by a whole block
 constexpr int kRowThreads = 1024;
 constexpr int kDeltaBlocks = 4096;  // k_delta_lb's largest grid (4 waves per block)
+// Edges of a batch above the one-block size whose lists are both long and of
+// similar length are intersected by merge path, in segments of kMergeSeg
+// merge positions, instead of one binary search per entry.
+constexpr int32_t kMergeMin = 512;    // the shorter list at least this long
+constexpr int32_t kMergeRatio = 8;    // the longer at most this many times as long
+constexpr int32_t kMergeSeg = 4096;
 // The insertion's pool reservation, words of a device control block: the
 // pool's first free entry and capacity (set by the host per batch), whether
 // the rows that move fit (else no row is touched, and the host grows the
@@ -826,16 +832,36 @@ __global__ void k_set_ctl(unsigned long long *__restrict__ ctl, unsigned long lo
 // between two hubs is spread over many waves instead of one.
 // (items 0 past the count *nd, and everywhere when the merges of this
 // insertion did not run: ctl says so)
+// With msegs given, an edge whose lists are both long and of similar length
+// (merge_edge) gets no items but msegs[t] merge-path segments of kMergeSeg
+// positions (k_delta_merge).
+__device__ inline bool merge_edge(int32_t mn, int32_t mx)
+{
+    return mn >= kMergeMin && (int64_t)mx <= (int64_t)kMergeRatio * mn;
+}
+
 __global__ void k_delta_items(const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
                               const int32_t *__restrict__ sel, int64_t n,
                               const int64_t *__restrict__ nd,
                               const unsigned long long *__restrict__ ctl,
-                              const int32_t *__restrict__ len, int32_t *__restrict__ items)
+                              const int32_t *__restrict__ len, int32_t *__restrict__ items,
+                              int32_t *__restrict__ msegs)
 {
     const int64_t lim = (ctl && !ctl[kCtlFits]) ? 0 : bound(n, nd);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x)
-        items[t] = (t < lim && (sel == nullptr || sel[t])) ? min(len[pa[t]], len[pb[t]]) : 0;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int32_t it = 0, sg = 0;
+        if (t < lim && (sel == nullptr || sel[t])) {
+            const int32_t la = len[pa[t]], lb = len[pb[t]];
+            const int32_t mn = min(la, lb), mx = max(la, lb);
+            if (msegs && merge_edge(mn, mx))
+                sg = (int32_t)ceil_div((int64_t)mn + mx, kMergeSeg);
+            else
+                it = mn;
+        }
+        items[t] = it;
+        if (msegs) msegs[t] = sg;
+    }
 }
 
 // Largest t in [lo, hi] with pre[t] <= i.
@@ -940,6 +966,85 @@ __global__ __launch_bounds__(kThreads) void k_delta_lb(
         }
         finish(a, lo0);
         if (two) finish(b, lo1);
+    }
+    const unsigned long long part[1] = {(unsigned long long)local};
+    block_add_u64<1>(part, gsum);  // two's complement: a negative sum wraps
+}
+
+// The merge edges' delta.  Segment k of edge t covers merge positions
+// [k S, (k+1) S) of its shorter list X and longer list Y (S = kMergeSeg,
+// X first on ties); lanes 0 and 1 find where the two diagonals cross the
+// lists (the entries of X among the first d positions, by binary search).
+// The segment's X entries meet their equals, if any, in Y[iy0 .. iy1]: an
+// entry x = X[i] at merge position i + j (j entries of Y below it) lies in
+// the segment, so iy0 <= j <= iy1.  The parts are merged window by window by
+// the whole wave: each lane finds its x's co-rank in the 64-entry Y window
+// by a six-step search over the lanes; only values up to m = min(last x,
+// last y) are settled by a pair of windows, and each window then moves past
+// its entries <= m.  Every x is in one segment, so every common neighbour is
+// counted once; its credits are those of k_delta_lb.  Called per wave.
+__global__ __launch_bounds__(kThreads) void k_delta_merge(
+    const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+    const int64_t *__restrict__ mpre, int64_t n, const int32_t *__restrict__ pool,
+    const int64_t *__restrict__ boff, const int32_t *__restrict__ len,
+    const uint8_t *__restrict__ flag, int32_t sign, uint32_t *__restrict__ out,
+    uint32_t *__restrict__ ecnt, unsigned long long *__restrict__ gsum)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t total = mpre[n];
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    int64_t local = 0;
+    for (int64_t g = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; g < total;
+         g += nwaves) {
+        const int64_t t = edge_of(mpre, g, 0, n - 1);
+        const int64_t d0 = (g - mpre[t]) * kMergeSeg;
+        const int32_t s = pa[t], d = pb[t];
+        const int32_t ls = len[s], ld = len[d];
+        const bool s_short = ls <= ld;
+        const int32_t nx = s_short ? ls : ld, ny = s_short ? ld : ls;
+        const int64_t xb = s_short ? boff[s] : boff[d], yb = s_short ? boff[d] : boff[s];
+        const int32_t *X = pool + xb, *Y = pool + yb;
+        const int64_t d1 = min(d0 + kMergeSeg, (int64_t)nx + ny);
+        int32_t split = 0;
+        if (lane < 2) {  // X entries among the first dd merge positions
+            const int64_t dd = lane ? d1 : d0;
+            int32_t lo = (int32_t)max<int64_t>(0, dd - ny), hi = (int32_t)min<int64_t>(dd, nx);
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (X[mid] <= Y[dd - 1 - mid]) lo = mid + 1; else hi = mid;
+            }
+            split = lo;
+        }
+        int32_t ix = __shfl(split, 0, kWave);
+        const int32_t ix1 = __shfl(split, 1, kWave);
+        int32_t iy = (int32_t)(d0 - ix);
+        const int32_t iy1 = (int32_t)min<int64_t>(d1 - ix1 + 1, ny);  // (one past iy1, inclusive)
+        uint32_t msum = 0;
+        while (ix < ix1 && iy < iy1) {
+            const int32_t cx = min(kWave, ix1 - ix), cy = min(kWave, iy1 - iy);
+            const int32_t x = lane < cx ? X[ix + lane] : INT32_MAX;
+            const int32_t y = lane < cy ? Y[iy + lane] : INT32_MAX;
+            const int32_t m = min(__shfl(x, cx - 1, kWave), __shfl(y, cy - 1, kWave));
+            int32_t r = 0;  // window entries y < x among lanes 0..62
+#pragma unroll
+            for (int st = kWave / 2; st > 0; st >>= 1) {
+                const int32_t q = __shfl(y, r + st - 1, kWave);
+                r += (q < x) ? st : 0;
+            }
+            const int32_t yr = __shfl(y, r, kWave);  // (every lane takes part)
+            if (x <= m && yr == x) {
+                const int fs = flag[xb + ix + lane], fl = flag[yb + iy + r];
+                const int bs = s_short ? fs : fl, bd = s_short ? fl : fs;
+                const int cw = 2 - bs - bd;
+                if (cw) atomicAdd(&out[x], (uint32_t)(sign * cw));
+                local += sign * cw;
+                msum += (bs | bd) ? 1u : 2u;
+            }
+            ix += __popcll(__ballot(x <= m));
+            iy += __popcll(__ballot(y <= m));
+        }
+        msum = wave_sum<uint32_t>(msum);
+        if (lane == 0 && msum) atomicAdd(&ecnt[t], msum);
     }
     const unsigned long long part[1] = {(unsigned long long)local};
     block_add_u64<1>(part, gsum);  // two's complement: a negative sum wraps
@@ -1460,13 +1565,23 @@ int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel
     int32_t *items = g.lbuf[0].as<int32_t>();
     int64_t *pre = g.lbuf[1].as<int64_t>();
     uint32_t *ecnt = g.lbuf[2].as<uint32_t>();
+    const bool merges = n > kSmallPairs;  // (small batches search every entry)
+    int32_t *msegs = nullptr;
+    int64_t *mpre = nullptr;
+    if (merges) {
+        TC_TRY(g.lbuf[3].ensure((size_t)n * sizeof(int32_t)));
+        TC_TRY(g.lbuf[4].ensure((size_t)(n + 1) * sizeof(int64_t)));
+        msegs = g.lbuf[3].as<int32_t>();
+        mpre = g.lbuf[4].as<int64_t>();
+    }
     if (n <= kSmallPairs) {
         hipLaunchKernelGGL(k_small_delta_prep, dim3(1), dim3(1024), 0, s, pa, pb, sel, n, nd, ctl,
                            g.st.len.as<int32_t>(), pre, ecnt);
     } else {
         hipLaunchKernelGGL(k_delta_items, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, pa, pb, sel,
-                           n, nd, ctl, g.st.len.as<int32_t>(), items);
+                           n, nd, ctl, g.st.len.as<int32_t>(), items, msegs);
         TC_TRY(scan_counts_i32(items, pre, n, g.tmp, s));
+        TC_TRY(scan_counts_i32(msegs, mpre, n, g.tmp, s));
         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, ecnt, n);
     }
     // a grid by the batch size (the item total is known on the device only;
@@ -1475,6 +1590,10 @@ int run_delta(Graph &g, const int32_t *pa, const int32_t *pb, const int32_t *sel
     hipLaunchKernelGGL(k_delta_lb, dim3(dblocks), dim3(kThreads), 0, s, pa, pb, pre, n,
                        g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                        g.flagpool.as<uint8_t>(), sign, out, ecnt, gsum);
+    if (merges)
+        hipLaunchKernelGGL(k_delta_merge, dim3(dblocks), dim3(kThreads), 0, s, pa, pb, mpre, n,
+                           g.st.pool.as<int32_t>(), g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
+                           g.flagpool.as<uint8_t>(), sign, out, ecnt, gsum);
     hipLaunchKernelGGL(k_delta_edges, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, pa, pb, n, ecnt,
                        sign, out, gsum);
     TC_HIP(hipGetLastError());
@@ -1597,6 +1716,10 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
     TC_TRY(g.lbuf[0].ensure((size_t)Bc * sizeof(int32_t)));
     TC_TRY(g.lbuf[1].ensure((size_t)(Bc + 1) * sizeof(int64_t)));
     TC_TRY(g.lbuf[2].ensure((size_t)Bc * sizeof(uint32_t)));
+    if (Bc > kSmallPairs) {
+        TC_TRY(g.lbuf[3].ensure((size_t)Bc * sizeof(int32_t)));
+        TC_TRY(g.lbuf[4].ensure((size_t)(Bc + 1) * sizeof(int64_t)));
+    }
     // device counts
     const int64_t *nI = pi + Bc, *nD = pd + Bc;  // unique insertions / deletions
     const int64_t *nNew = sidx + Bc;              // insertions of absent pairs
@@ -1758,7 +1881,7 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         mix(kv, sizeof(kv));
         const void *ptrs[] = {out, g.st.pool.p, g.flagpool.p, g.st.boff.p, g.st.len.p, g.st.cap.p,
                               g.vmark.p, g.accum.p, g.tmp.p, g.bsrc.p, g.bdst.p, g.bdel.p, g.boffb.p,
-                              g.lbuf[0].p, g.lbuf[1].p, g.lbuf[2].p};
+                              g.lbuf[0].p, g.lbuf[1].p, g.lbuf[2].p, g.lbuf[3].p, g.lbuf[4].p};
         mix(ptrs, sizeof(ptrs));
         for (const DevBuf &b : g.sb) mix(&b.p, sizeof(b.p));
         key |= 1ull;
