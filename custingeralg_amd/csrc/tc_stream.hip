@@ -32,9 +32,18 @@ constexpr int kDeltaBlocks = 4096;  // k_delta_lb's largest grid (4 waves per bl
 // Edges of a batch above the one-block size whose lists are both long and of
 // similar length are intersected by merge path, in segments of kMergeSeg
 // merge positions, instead of one binary search per entry.
-constexpr int32_t kMergeMin = 512;    // the shorter list at least this long
-constexpr int32_t kMergeRatio = 8;    // the longer at most this many times as long
-constexpr int32_t kMergeSeg = 4096;
+#ifndef TC_MERGE_MIN
+#define TC_MERGE_MIN 256
+#endif
+#ifndef TC_MERGE_RATIO
+#define TC_MERGE_RATIO 16
+#endif
+#ifndef TC_MERGE_SEG
+#define TC_MERGE_SEG 4096
+#endif
+constexpr int32_t kMergeMin = TC_MERGE_MIN;      // the shorter list at least this long
+constexpr int32_t kMergeRatio = TC_MERGE_RATIO;  // the longer at most this many times as long
+constexpr int32_t kMergeSeg = TC_MERGE_SEG;
 // The insertion's pool reservation, words of a device control block: the
 // pool's first free entry and capacity (set by the host per batch), whether
 // the rows that move fit (else no row is touched, and the host grows the
