@@ -319,6 +319,56 @@ def test_hub_rows_merge_and_compact(slack, n, hub_deg):
     g.close()
 
 
+def test_merge_path_delta_against_oracle():
+    """A batch above the one-block size whose hub-hub pairs (both rows 3,000+
+    entries, two merge-path segments each) go through the merge-path delta
+    and whose other pairs through the per-entry searches: the per-vertex
+    delta and the store against the oracle, for the insertions and for the
+    deletions of the same pairs."""
+    n, hubs = 20000, 16
+    rng = np.random.default_rng(41)
+    bu = rng.integers(0, n, 60000)
+    bv = rng.integers(0, n, 60000)
+    hu = np.repeat(np.arange(hubs), 3000)
+    hv = np.concatenate([rng.choice(np.arange(hubs, n), 3000, replace=False) for _ in range(hubs)])
+    pairs = [(a, b) for a in range(hubs) for b in range(a + 1, hubs)]
+    present = [p for i, p in enumerate(pairs) if i % 2 == 0]
+    absent = [p for i, p in enumerate(pairs) if i % 2 == 1]
+    a = np.concatenate([bu, hu, [p[0] for p in present]]).astype(np.int32)
+    b = np.concatenate([bv, hv, [p[1] for p in present]]).astype(np.int32)
+    k = a != b
+    off, adj = O.apply_batch_csr(n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32),
+                                 ins_u=a[k], ins_v=b[k])
+    assert np.diff(off)[:hubs].min() >= 3000
+    g = T.cuStinger(off, adj)
+    delta = T.DeviceArray(n, np.uint32).zero()
+    try:
+        base = O.custinger_count(off, adj)
+        # insert the absent hub pairs and 1,500 background pairs
+        xu = rng.integers(hubs, n, 1500).astype(np.int32)
+        xv = rng.integers(hubs, n, 1500).astype(np.int32)
+        ku = xu != xv
+        u = np.concatenate([[p[0] for p in absent], xu[ku]]).astype(np.int32)
+        v = np.concatenate([[p[1] for p in absent], xv[ku]]).astype(np.int32)
+        dl = np.zeros(len(u), np.uint8)
+        assert len(u) > 1024
+        dg, _ = T.batch_apply(g, u, v, dl, delta)
+        cur_off, cur_adj = _host_apply(n, off, adj, u, v, dl)
+        got = g.to_csr()
+        assert np.array_equal(got[0], cur_off) and np.array_equal(got[1], cur_adj)
+        now = O.custinger_count(cur_off, cur_adj)
+        assert np.array_equal(base + delta.to_host(), now)
+        assert dg == int(now.astype(np.int64).sum()) - int(base.astype(np.int64).sum())
+        # delete the same pairs again: back to the start
+        dg2, _ = T.batch_apply(g, u, v, np.ones(len(u), np.uint8), delta)
+        got = g.to_csr()
+        assert np.array_equal(got[0], off) and np.array_equal(got[1], adj)
+        assert not delta.to_host().any() and dg2 == -dg
+    finally:
+        delta.free()
+        g.close()
+
+
 def test_large_batch_matches_two_halves():
     """A batch of over 2^18 pairs, a third of them at six hub rows of 9,000+
     entries, against the same batch applied to a twin store in two halves:
