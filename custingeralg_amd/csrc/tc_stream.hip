@@ -484,13 +484,15 @@ __global__ __launch_bounds__(256) void k_compact_rows(const int32_t *__restrict_
 __global__ __launch_bounds__(kRowThreads) void k_compact_rows_block(
     const int32_t *__restrict__ queue, int64_t n0, const int64_t *__restrict__ nd,
     const int64_t *__restrict__ boff,
-    int32_t *__restrict__ len, int32_t *__restrict__ pool, uint8_t *__restrict__ flag)
+    int32_t *__restrict__ len, int32_t *__restrict__ pool, uint8_t *__restrict__ flag,
+    const int32_t *__restrict__ big, const unsigned long long *__restrict__ nbig)
 {
     __shared__ int32_t cnt[kRowPerBlock * (kRowThreads / kWave)];
     __shared__ int32_t pre[kRowPerBlock * (kRowThreads / kWave) + 1];
-    const int64_t n = bound(n0, nd);
-    for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
-        const int32_t v = queue[q];
+    // (with big given: only the queue entries k_select_big listed)
+    const int64_t n = big ? (int64_t)*nbig : bound(n0, nd);
+    for (int64_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const int32_t v = queue[big ? big[k] : k];
         const int32_t l = len[v];
         if (l < kBigRow) continue;
         const int32_t w = compact_row<kRowThreads, kRowPerBlock>(boff[v], l, pool, flag, cnt, pre);
@@ -753,6 +755,31 @@ __global__ __launch_bounds__(256) void k_merge_rows(
     }
 }
 
+// The entries i < *nd (n without nd) of a row list (store rows, or the
+// rows of merge segments) whose row holds kBigRow entries or more, appended
+// to out (order arbitrary) with their number added to *cnt (zeroed before):
+// the block kernels then visit only those instead of every entry.
+__global__ void k_select_big(const int32_t *__restrict__ rows, int64_t n0,
+                             const int64_t *__restrict__ nd, const int32_t *__restrict__ len,
+                             int32_t *__restrict__ out, unsigned long long *__restrict__ cnt)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int64_t n = bound(n0, nd);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); i0 < n;
+         i0 += stride) {
+        const int64_t i = i0 + lane;
+        const bool big = i < n && len[rows[i]] >= kBigRow;
+        const uint64_t m = __ballot(big);
+        if (!m) continue;  // (the whole wave)
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, kWave);
+        if (big) out[base + __popcll(m & lt)] = (int32_t)i;
+    }
+}
+
 // k_merge_rows for rows of at least kBigRow entries: one block per row.
 __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
     const int32_t *__restrict__ sv, const int64_t *__restrict__ sbeg, int64_t nseg0,
@@ -761,7 +788,8 @@ __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
     const int64_t *__restrict__ nscan, const unsigned long long *__restrict__ ctl,
     int64_t *__restrict__ boff,
     int32_t *__restrict__ len, int32_t *__restrict__ cap, int32_t *__restrict__ pool,
-    uint8_t *__restrict__ flag, int64_t *__restrict__ newpos)
+    uint8_t *__restrict__ flag, int64_t *__restrict__ newpos, const int32_t *__restrict__ big,
+    const unsigned long long *__restrict__ nbig)
 {
     constexpr int kKeys = 4096;
     __shared__ int32_t ys[kRowThreads * kRowPerBlock];
@@ -771,7 +799,10 @@ __global__ __launch_bounds__(kRowThreads) void k_merge_rows_block(
     const int64_t nseg = bound(nseg0, nsegd), n = bound(n0, nd);
     const int64_t pool_used = (int64_t)ctl[kCtlBase];
     const uint64_t m = (1ull << bv) - 1;
-    for (int64_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+    // (with big given: only the segments k_select_big listed)
+    const int64_t nvisit = big ? (int64_t)*nbig : nseg;
+    for (int64_t k = blockIdx.x; k < nvisit; k += gridDim.x) {
+        const int64_t s = big ? big[k] : k;
         const int32_t v = sv[s];
         const int32_t lo = len[v];
         if (lo < kBigRow) continue;
@@ -1763,10 +1794,17 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         // the long rows first: a row merged by the wave kernel may grow past
         // kBigRow, and the block kernel, which picks its rows by their length,
         // would then merge it again
+        // batches above the one-block size list their long rows first
+        if (!small) {
+            hipLaunchKernelGGL(k_zero_u64s, dim3(1), dim3(64), 0, st, acc + 28, 1);
+            hipLaunchKernelGGL(k_select_big, dim3(gB2), dim3(256), 0, st, sv, B2, nseg,
+                               g.st.len.as<int32_t>(), head, acc + 28);
+        }
         hipLaunchKernelGGL(k_merge_rows_block, dim3(grid_for(B2, 2, 1024)), dim3(kRowThreads), 0, st,
                            sv, sbeg, B2, nseg, B2, ndir, dk, bv, need, nscan, ctl,
                            g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(),
-                           g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>(), newpos);
+                           g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>(), newpos,
+                           small ? nullptr : head, small ? nullptr : acc + 28);
         hipLaunchKernelGGL(k_merge_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, st, sv, sbeg, B2,
                            nseg, B2, ndir, dk, bv, need, nscan, ctl, g.st.boff.as<int64_t>(),
                            g.st.len.as<int32_t>(), g.st.cap.as<int32_t>(), g.st.pool.as<int32_t>(),
@@ -1827,9 +1865,13 @@ int batch_apply(Graph &g, const int32_t *u, const int32_t *v, const uint8_t *is_
         hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(B2, 4, 65536)), dim3(256), 0, st, queue, B2,
                            qlen, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
                            g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
+        if (!small)  // (acc[27] is zero from the start of the batch)
+            hipLaunchKernelGGL(k_select_big, dim3(gB2), dim3(256), 0, st, queue, B2, qlen,
+                               g.st.len.as<int32_t>(), head, acc + 27);
         hipLaunchKernelGGL(k_compact_rows_block, dim3(grid_for(B2, 2, 1024)), dim3(kRowThreads), 0, st,
                            queue, B2, qlen, g.st.boff.as<int64_t>(), g.st.len.as<int32_t>(),
-                           g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>());
+                           g.st.pool.as<int32_t>(), g.flagpool.as<uint8_t>(),
+                           small ? nullptr : head, small ? nullptr : acc + 27);
         hipLaunchKernelGGL(k_unmark, dim3(gB2), dim3(256), 0, st, queue, B2, qlen,
                            g.vmark.as<int32_t>());
         TC_HIP(hipGetLastError());
