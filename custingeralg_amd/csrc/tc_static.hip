@@ -1874,6 +1874,9 @@ template <int LMAX, int NW, bool ORIENTED>
 #ifndef TC_MID_WAVES
 #define TC_MID_WAVES 6
 #endif
+#ifndef TC_MID_LOOKUPS
+#define TC_MID_LOOKUPS 1
+#endif
 __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_WAVES : 1) void k_count(const Job *__restrict__ jobs, int64_t njobs,
                                                       CountArgs A)
 {
@@ -1989,10 +1992,11 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
             if (use_bm) {
                 BT.find_n(w, slot);
             } else {
-                // lookups in flight: four in the small bins; two in the largest
-                // bin, where the table is the fallback of the bitmap and
-                // registers are scarce
-                constexpr int gmax = LMAX >= 4096 ? 2 : 4;
+                // lookups in flight: four in the small bin; one in the mid bin
+                // (its spills 112 -> 44 B per lane, bin 19.9 -> 19.4 ms at
+                // R-MAT 24); two in the largest bin, where the table is the
+                // fallback of the bitmap and registers are scarce
+                constexpr int gmax = LMAX >= 4096 ? 2 : (LMAX >= 512 ? TC_MID_LOOKUPS : 4);
                 constexpr int g = n < gmax ? n : gmax;
 #pragma unroll
                 for (int u = 0; u < n; u += g) T.find_n<g>(&w[u], &slot[u]);
