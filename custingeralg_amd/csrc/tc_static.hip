@@ -58,15 +58,42 @@ __device__ inline int64_t readlane_i64(int64_t x, int lane)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Bucketed two-choice hash table in LDS.  Bucket b holds up to four ids in
-// slots 4b..4b+3, filled as a prefix (inserts claim the first empty slot by
-// CAS), -1 when empty.  An id goes to its first bucket, to its second when
-// the first is full, and when both are full to the next non-full bucket after
-// the first.  A lookup reads the first bucket (one 16-byte LDS read), the
-// second only when the first is full, and loops over further buckets only
-// when both are full; the load factor <= 1/2 makes each step rarer.
-struct BucketTable {
-    int4 *HB;
+// Bucketed two-choice hash table in LDS.  Bucket b holds up to S ids (S = 4:
+// 16-byte buckets, S = 2: 8-byte buckets) in slots S b .. S b + S - 1, filled
+// as a prefix (inserts claim the first empty slot by CAS), -1 when empty.  An
+// id goes to its first bucket, to its second when the first is full, and
+// when both are full to the next non-full bucket after the first.  A lookup
+// reads the first bucket (one LDS read), the second only when the first is
+// full, and loops over further buckets only when both are full; the load
+// factor <= 1/2 makes each step rarer.
+template <int S> struct BucketWord;
+template <> struct BucketWord<4> {
+    using T = int4;
+    __device__ static T empty() { return make_int4(-1, -1, -1, -1); }
+    __device__ static int32_t last(const T &v) { return v.w; }
+    __device__ static int32_t slot_in(const T &v, int32_t w)
+    {
+        const int32_t m0 = v.x == w, m1 = v.y == w, m2 = v.z == w, m3 = v.w == w;
+        return (m0 | m1 | m2 | m3) ? m1 + 2 * m2 + 3 * m3 : -1;
+    }
+};
+template <> struct BucketWord<2> {
+    using T = int2;
+    __device__ static T empty() { return make_int2(-1, -1); }
+    __device__ static int32_t last(const T &v) { return v.y; }
+    __device__ static int32_t slot_in(const T &v, int32_t w)
+    {
+        const int32_t m0 = v.x == w, m1 = v.y == w;
+        return (m0 | m1) ? m1 : -1;
+    }
+};
+
+template <int S>
+struct BucketTableT {
+    using W = BucketWord<S>;
+    using V = typename W::T;
+    static constexpr int kLog = S == 4 ? 2 : 1;
+    V *HB;
     int nbb;  // log2(number of buckets)
 
     __device__ inline uint32_t b1(int32_t w) const
@@ -83,7 +110,7 @@ struct BucketTable {
     {
         int *s = reinterpret_cast<int *>(&HB[b]);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < S; ++k)
             if (atomicCAS(&s[k], -1, w) == -1) return true;
         return false;
     }
@@ -102,25 +129,24 @@ struct BucketTable {
             if (claim(b, w)) return;
     }
     // Slot of w in bucket b or -1; arithmetic only (ids in a bucket are distinct).
-    __device__ inline int32_t match(const int4 v, uint32_t b, int32_t w) const
+    __device__ inline int32_t match(const V v, uint32_t b, int32_t w) const
     {
-        const int32_t m0 = v.x == w, m1 = v.y == w, m2 = v.z == w, m3 = v.w == w;
-        const int32_t idx = m1 + 2 * m2 + 3 * m3;
-        return (m0 | m1 | m2 | m3) ? (int32_t)(b << 2) + idx : -1;
+        const int32_t k = W::slot_in(v, w);
+        return k >= 0 ? (int32_t)(b << kLog) + k : -1;
     }
     // The rest of a lookup whose first bucket vx (index x) is full and does
     // not hold w: the second bucket, then the probe sequence if both are full.
     __device__ inline int32_t find_rest(uint32_t x, int32_t w) const
     {
         const uint32_t y = b2(w);
-        const int4 vy = HB[y];
+        const V vy = HB[y];
         int32_t s = match(vy, y, w);
-        if (s < 0 && vy.w >= 0) {
+        if (s < 0 && W::last(vy) >= 0) {
             const uint32_t m = (1u << nbb) - 1;
             for (uint32_t b = (x + 1) & m;; b = (b + 1) & m) {
-                const int4 v = HB[b];
+                const V v = HB[b];
                 s = match(v, b, w);
-                if (s >= 0 || v.w < 0) break;
+                if (s >= 0 || W::last(v) < 0) break;
             }
         }
         return s;
@@ -129,9 +155,9 @@ struct BucketTable {
     __device__ inline int32_t find(int32_t w) const
     {
         const uint32_t x = b1(w);
-        const int4 vx = HB[x];
+        const V vx = HB[x];
         int32_t s = match(vx, x, w);
-        if (s < 0 && w >= 0 && vx.w >= 0) s = find_rest(x, w);
+        if (s < 0 && w >= 0 && W::last(vx) >= 0) s = find_rest(x, w);
         return w >= 0 ? s : -1;  // empty slots hold -1
     }
     // Batched find: the first buckets of every value are read before any is
@@ -139,23 +165,23 @@ struct BucketTable {
     template <int U>
     __device__ inline void find_n(const int32_t *w, int32_t *slot) const
     {
-        int4 vx[U];
+        V vx[U];
         uint32_t x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             x[u] = b1(w[u]);
             vx[u] = HB[x[u]];
         }
-        int4 vy[U];
+        V vy[U];
         uint32_t y[U];
         int32_t s[U];
         bool need[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             s[u] = match(vx[u], x[u], w[u]);
-            need[u] = s[u] < 0 && w[u] >= 0 && vx[u].w >= 0;
+            need[u] = s[u] < 0 && w[u] >= 0 && W::last(vx[u]) >= 0;
             y[u] = 0;
-            vy[u] = make_int4(-1, -1, -1, -1);
+            vy[u] = W::empty();
             if (need[u]) {  // (these reads are in flight together)
                 y[u] = b2(w[u]);
                 vy[u] = HB[y[u]];
@@ -165,12 +191,12 @@ struct BucketTable {
         for (int u = 0; u < U; ++u) {
             if (need[u]) {
                 s[u] = match(vy[u], y[u], w[u]);
-                if (s[u] < 0 && vy[u].w >= 0) {
+                if (s[u] < 0 && W::last(vy[u]) >= 0) {
                     const uint32_t m = (1u << nbb) - 1;
                     for (uint32_t b = (x[u] + 1) & m;; b = (b + 1) & m) {
-                        const int4 v = HB[b];
+                        const V v = HB[b];
                         s[u] = match(v, b, w[u]);
-                        if (s[u] >= 0 || v.w < 0) break;
+                        if (s[u] >= 0 || W::last(v) < 0) break;
                     }
                 }
             }
@@ -182,6 +208,7 @@ struct BucketTable {
         return reinterpret_cast<const int32_t *>(HB)[slot];
     }
 };
+using BucketTable = BucketTableT<4>;
 
 // Membership bitmap of a sorted owner tile over its value range [lo, lo+span):
 // word i holds the bits of values lo+32i .. lo+32i+31 (x) and the number of
@@ -1877,13 +1904,19 @@ template <int LMAX, int NW, bool ORIENTED>
 #ifndef TC_MID_LOOKUPS
 #define TC_MID_LOOKUPS 1
 #endif
+#ifndef TC_SMALL_SLOTS
+#define TC_SMALL_SLOTS 2
+#endif
+#ifndef TC_MID_SLOTS
+#define TC_MID_SLOTS 2
+#endif
 __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_WAVES : 1) void k_count(const Job *__restrict__ jobs, int64_t njobs,
                                                       CountArgs A)
 {
-    // The owner tile lives in a bucketed two-choice hash table: 4 ids per
-    // 16-byte bucket, load <= 1/2, so a lookup is two ds_read_b128 and eight
-    // compares with no probe loop unless both buckets are full.  C counts hits
-    // per slot (third-vertex counts of the oriented method).
+    // The owner tile lives in a bucketed two-choice hash table (BucketTableT),
+    // load <= 1/2, so a lookup reads one bucket, a second only where the
+    // first is full, and loops only when both are.  C counts hits per slot
+    // (third-vertex counts of the oriented method).
     constexpr int NBMAX = (LMAX + 1) / 2 > 0 ? (LMAX + 1) / 2 : 1;
     __shared__ int4 HB[NBMAX];
     __shared__ uint32_t C[ORIENTED ? 4 * NBMAX : 1];
@@ -1920,9 +1953,15 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
         const int32_t L = min(LMAX, Lfull - s0);
         const bool tiled = LMAX >= 4096 && Lfull > LMAX;  // smaller bins hold whole lists
         const int32_t *Sg = A.adj + obase + s0;  // owner tile (sorted) in global memory
+        // 2 ids per bucket in the small and mid bins (an 8-byte read per
+        // probe; the small bin 3.8 -> 3.5 ms at R-MAT 24, 36.2 -> 34.8 ms on
+        // 4096 batched graphs, the mid bin within noise), 4 in the largest;
+        // either way at least 2L slots
+        constexpr int kSlots = LMAX <= 64 ? TC_SMALL_SLOTS : LMAX <= 512 ? TC_MID_SLOTS : 4;
+        using Table = BucketTableT<kSlots>;
         int nbb = 0;
-        while ((1 << nbb) * 2 < L) ++nbb;         // buckets: 2^nbb >= L/2
-        const BucketTable T{HB, nbb};
+        while ((kSlots << nbb) < 2 * L) ++nbb;
+        const Table T{reinterpret_cast<typename Table::V *>(HB), nbb};
         const int32_t lo_v = Sg[0], hi_v = Sg[L - 1];
         // a tile whose value range is narrow enough gets a bitmap instead of
         // the hash table (same LDS space)
@@ -1979,9 +2018,9 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
             }
             __syncthreads();
         } else {
-            for (int i = threadIdx.x; i < (1 << nbb); i += nthr) HB[i] = make_int4(-1, -1, -1, -1);
+            for (int i = threadIdx.x; i < (1 << nbb); i += nthr) T.HB[i] = Table::W::empty();
             if (ORIENTED)
-                for (int i = threadIdx.x; i < 4 << nbb; i += nthr) C[i] = 0;
+                for (int i = threadIdx.x; i < kSlots << nbb; i += nthr) C[i] = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
             __syncthreads();
@@ -1999,7 +2038,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
                 constexpr int gmax = LMAX >= 4096 ? 2 : (LMAX >= 512 ? TC_MID_LOOKUPS : 4);
                 constexpr int g = n < gmax ? n : gmax;
 #pragma unroll
-                for (int u = 0; u < n; u += g) T.find_n<g>(&w[u], &slot[u]);
+                for (int u = 0; u < n; u += g) T.template find_n<g>(&w[u], &slot[u]);
             }
         };
         auto lookup = [&](int32_t w) -> int32_t {
@@ -2283,7 +2322,7 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
                     if (ci) atomicAdd(&A.out[A.order[Sg[i]]], 2u * ci);
                 }
             } else {
-                for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
+                for (int i = threadIdx.x; i < kSlots << nbb; i += nthr) {
                     const uint32_t ci = C[i];
                     if (ci) atomicAdd(&A.out[A.order[T.id_at(i)]], 2u * ci);
                 }
