@@ -1910,6 +1910,9 @@ template <int LMAX, int NW, bool ORIENTED>
 #ifndef TC_MID_SLOTS
 #define TC_MID_SLOTS 2
 #endif
+#ifndef TC_LONG_SLOTS
+#define TC_LONG_SLOTS 4
+#endif
 __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_WAVES : 1) void k_count(const Job *__restrict__ jobs, int64_t njobs,
                                                       CountArgs A)
 {
@@ -2418,9 +2421,11 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             if (BM && A.hjobs && threadIdx.x == 0) A.hjobs[atomicAdd(A.job_ctr + 1, 1ull)] = jb;
             continue;
         }
+        using Table = BucketTableT<TC_LONG_SLOTS>;
+        constexpr int kSlots = TC_LONG_SLOTS;
         int nbb = 0;
-        while ((1 << nbb) * 2 < L) ++nbb;
-        const BucketTable T{HB, nbb};
+        while ((kSlots << nbb) < 2 * L) ++nbb;
+        const Table T{reinterpret_cast<typename Table::V *>(HB), nbb};
         uint2 *BMw = reinterpret_cast<uint2 *>(RAW);
         // bitmap: always in the bitmap launch, and in the hash-table launch
         // when the span fits the wide layout (counters after the words)
@@ -2450,8 +2455,8 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
             }
             __syncthreads();
         } else {
-            for (int i = threadIdx.x; i < (1 << nbb); i += nthr) HB[i] = make_int4(-1, -1, -1, -1);
-            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) C[i] = 0;
+            for (int i = threadIdx.x; i < (1 << nbb); i += nthr) T.HB[i] = Table::W::empty();
+            for (int i = threadIdx.x; i < kSlots << nbb; i += nthr) C[i] = 0;
             __syncthreads();
             for (int i = threadIdx.x; i < L; i += nthr) T.insert(Sg[i]);
             __syncthreads();
@@ -2501,7 +2506,7 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
                 } else {
                     int32_t slots[kLongUnroll];
 #pragma unroll
-                    for (int u = 0; u < kLongUnroll; u += 2) T.find_n<2>(&w[u], &slots[u]);
+                    for (int u = 0; u < kLongUnroll; u += 2) T.template find_n<2>(&w[u], &slots[u]);
 #pragma unroll
                     for (int u = 0; u < kLongUnroll; ++u) {
                         const bool hit = slots[u] >= 0;
@@ -2531,7 +2536,7 @@ __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job
                 if (ci) atomicAdd(&A.outr[Sg[i]], 2u * ci);
             }
         } else {
-            for (int i = threadIdx.x; i < 4 << nbb; i += nthr) {
+            for (int i = threadIdx.x; i < kSlots << nbb; i += nthr) {
                 const uint32_t ci = C[i];
                 if (ci) atomicAdd(&A.outr[T.id_at(i)], 2u * ci);
             }
