@@ -2358,7 +2358,10 @@ __global__ __launch_bounds__(NW * kWave, LMAX <= 64 ? 6 : LMAX <= 512 ? TC_MID_W
 // BM = false, which takes the hash-table and wide-bitmap tiles from that list;
 // k_count<LMAX, NW, true> with long_external does the tasks of the other
 // strategies.
-constexpr int kLongUnroll = 8;
+#ifndef TC_LONG_UNROLL
+#define TC_LONG_UNROLL 8
+#endif
+constexpr int kLongUnroll = TC_LONG_UNROLL;
 
 template <int LMAX, int NW, bool BM>
 __global__ __launch_bounds__(NW * kWave, BM ? 6 : 4) void k_count_long(const Job *__restrict__ jobs,
