@@ -1899,7 +1899,7 @@ __device__ inline void tile_filter(const int32_t *__restrict__ adj, int64_t &pb,
 // (64 VGPRs, about 60 spills).
 template <int LMAX, int NW, bool ORIENTED>
 #ifndef TC_MID_WAVES
-#define TC_MID_WAVES 6
+#define TC_MID_WAVES 7
 #endif
 #ifndef TC_MID_LOOKUPS
 #define TC_MID_LOOKUPS 1
@@ -2604,7 +2604,7 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
     int cus = 0;
     TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     // persistent grid: enough resident blocks to fill every CU
-    int per_cu = bin == 0 ? 4 * TC_SMALL_WAVES : (bin == 1 ? 12 : 2);
+    int per_cu = bin == 0 ? 4 * TC_SMALL_WAVES : (bin == 1 ? 2 * TC_MID_WAVES : 2);
     int64_t grid = std::min<int64_t>(njobs, (int64_t)cus * per_cu);
     if (bin == 0) {
         if (oriented)
