@@ -1893,10 +1893,11 @@ __device__ inline void tile_filter(const int32_t *__restrict__ adj, int64_t &pb,
     pl = lo - a0;
 }
 
-// The small and mid bins run at 6 waves per SIMD (at most 80 VGPRs; the
-// compiler spills about 45 values to scratch).  Measured at R-MAT 24 and on
-// 4096 batched graphs, this beats 4 waves (123 VGPRs, no spills) and 8 waves
-// (64 VGPRs, about 60 spills).
+// The small and mid bins run at 7 waves per SIMD (at most 72 VGPRs; the mid
+// bin spills a few values to scratch).  Measured at R-MAT 24 and on 4096
+// batched graphs, this beats 4 to 6 waves (fewer jobs in flight to hide
+// their dependent loads) and, in the mid bin, 8 (64 VGPRs, far more spills:
+// 45 -> 65 ms batched).
 template <int LMAX, int NW, bool ORIENTED>
 #ifndef TC_MID_WAVES
 #define TC_MID_WAVES 7
