@@ -28,7 +28,10 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kBigRow = 1024;   // rows this long are compacted / merged by a whole block
 constexpr int kRowThreads = 1024;
-constexpr int kDeltaBlocks = 4096;  // k_delta_lb's largest grid (4 waves per block)
+#ifndef TC_DELTA_BLOCKS
+#define TC_DELTA_BLOCKS 4096
+#endif
+constexpr int kDeltaBlocks = TC_DELTA_BLOCKS;  // k_delta_lb's largest grid (4 waves per block)
 // Edges of a batch above the one-block size whose lists are both long and of
 // similar length are intersected by merge path, in segments of kMergeSeg
 // merge positions, instead of one binary search per entry.
