@@ -634,10 +634,36 @@ __global__ void k_orient_write_plus_rank(const int64_t *__restrict__ coff,
     }
 }
 
-// One lane per rank row: N+ parts of 2..8 entries are sorted in registers by
-// a bitonic network, longer ones are listed by size class: (8,64] -> lst[0],
-// (64,512] -> lst[1], (512,4096] -> lst[2], (4096,16384] -> lst[3], above ->
-// lst[4].  List q starts at lst + off[q], its length is cnt[q].
+// One lane per rank row: N+ parts of 2..kThreadSort entries are sorted in
+// registers by a bitonic network (of 16 entries where a lane of the wave has
+// more than 8, else of 8), longer ones are listed by size class:
+// (kThreadSort,64] -> lst[0], (64,512] -> lst[1], (512,4096] -> lst[2],
+// (4096,16384] -> lst[3], above -> lst[4].  List q starts at lst + off[q],
+// its length is cnt[q].
+#ifndef TC_THREAD_SORT
+#define TC_THREAD_SORT 16
+#endif
+constexpr int kThreadSort = TC_THREAD_SORT;  // 8 or 16
+template <int N>
+__device__ inline void bitonic_regs(int32_t (&v)[N])
+{
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const int32_t a = v[i], b = v[l];
+                    v[i] = up ? min(a, b) : max(a, b);
+                    v[l] = up ? max(a, b) : min(a, b);
+                }
+            }
+        }
+    }
+}
 __global__ void k_sort_plus_small(const VInfo *__restrict__ vi, int32_t nv,
                                   int32_t *__restrict__ padj, int32_t *__restrict__ lst,
                                   const int64_t *__restrict__ off /* [5] */,
@@ -656,7 +682,7 @@ __global__ void k_sort_plus_small(const VInfo *__restrict__ vi, int32_t nv,
             x = vi[r];
             dp = x.len - x.dm;
         }
-        const int q = dp <= 8 ? -1
+        const int q = dp <= kThreadSort ? -1
                               : (dp <= 64 ? 0 : (dp <= 512 ? 1 : (dp <= 4096 ? 2 : (dp <= 16384 ? 3 : 4))));
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
@@ -667,35 +693,30 @@ __global__ void k_sort_plus_small(const VInfo *__restrict__ vi, int32_t nv,
             base = __shfl(base, __ffsll((unsigned long long)m) - 1, kWave);
             if (q == c) lst[off[c] + base + __popcll(m & lt)] = (int32_t)r;
         }
-        if (dp < 2 || dp > 8) continue;
+        const bool mine = dp >= 2 && dp <= kThreadSort;
         int32_t *P = padj + x.rb;
-        int32_t v[8];
+        if (kThreadSort > 8 && __ballot(mine && dp > 8)) {
+            int32_t v[kThreadSort > 8 ? kThreadSort : 8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = i < dp ? P[i] : INT32_MAX;
+            for (int i = 0; i < kThreadSort; ++i) v[i] = (mine && i < dp) ? P[i] : INT32_MAX;
+            bitonic_regs(v);
 #pragma unroll
-        for (int k = 2; k <= 8; k <<= 1) {
+            for (int i = 0; i < kThreadSort; ++i)
+                if (mine && i < dp) P[i] = v[i];
+        } else if (__ballot(mine)) {
+            int32_t v[8];
 #pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = 0; i < 8; ++i) v[i] = (mine && i < dp) ? P[i] : INT32_MAX;
+            bitonic_regs(v);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int l = i ^ j;
-                    if (l > i) {
-                        const bool up = (i & k) == 0;
-                        const int32_t a = v[i], b = v[l];
-                        v[i] = up ? min(a, b) : max(a, b);
-                        v[l] = up ? max(a, b) : min(a, b);
-                    }
-                }
-            }
+            for (int i = 0; i < 8; ++i)
+                if (mine && i < dp) P[i] = v[i];
         }
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (i < dp) P[i] = v[i];
     }
 }
 
-// N+ parts of 9..64 entries: one wavefront per listed row, bitonic network
-// across the lanes.
+// N+ parts of kThreadSort+1..64 entries: one wavefront per listed row,
+// bitonic network across the lanes.
 __global__ void k_sort_plus_wave(const VInfo *__restrict__ vi, const int32_t *__restrict__ rows,
                                  const int32_t *__restrict__ nrows, int32_t *__restrict__ padj)
 {
