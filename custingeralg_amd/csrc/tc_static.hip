@@ -2613,7 +2613,10 @@ constexpr int kL0 = 64, kL1 = 512, kL2 = 4096;
 // owners of at least this degree get their task lists from their own rows
 // (k_orient_transpose_hubs) instead of by appends
 constexpr int kHubDegree = 65536;  // with the appends
-constexpr int kNW0 = 1, kNW1 = 2, kNW2 = 8;
+#ifndef TC_MID_NW
+#define TC_MID_NW 2
+#endif
+constexpr int kNW0 = 1, kNW1 = TC_MID_NW, kNW2 = 8;
 
 // ev_long (bin 2, oriented): events recorded before and after the bitmap-tile
 // launch of k_count_long and after its hash-table-tile launch.
@@ -2626,7 +2629,7 @@ int launch_bin(int bin, bool oriented, const Job *jobs, int64_t njobs, const Cou
     int cus = 0;
     TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     // persistent grid: enough resident blocks to fill every CU
-    int per_cu = bin == 0 ? 4 * TC_SMALL_WAVES : (bin == 1 ? 2 * TC_MID_WAVES : 2);
+    int per_cu = bin == 0 ? 4 * TC_SMALL_WAVES / kNW0 : (bin == 1 ? 4 * TC_MID_WAVES / kNW1 : 2);
     int64_t grid = std::min<int64_t>(njobs, (int64_t)cus * per_cu);
     if (bin == 0) {
         if (oriented)
